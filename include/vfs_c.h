@@ -1,0 +1,227 @@
+/*
+ * vfs_c.h -- C ABI of the MI355X WCSPH substep engine (libvfs.so).
+ *
+ * Every entry point returns 0 on success and a negative vfs_status on failure;
+ * vfs_last_error() then returns a thread-local message.  Device work is
+ * stream-ordered on the caller's hipStream_t, passed as `void *stream`
+ * (NULL = the library's own stream for a vfs_sim, the null stream elsewhere).
+ * Host arrays are owned by the caller and copied; device buffers created by
+ * the library are freed by the matching *_destroy call.
+ *
+ * Reference interfaces replaced (luihabl/VkFluidSim, paths relative to the
+ * reference root) are cited per declaration.
+ */
+#ifndef VFS_C_H
+#define VFS_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFS_ABI_VERSION 1
+
+typedef enum vfs_status {
+    VFS_OK = 0,
+    VFS_ERR_INVALID = -1,   /* bad argument (null pointer, size out of range, ...) */
+    VFS_ERR_HIP = -2,       /* a HIP runtime call failed */
+    VFS_ERR_STATE = -3,     /* object not in a state that allows the call */
+    VFS_ERR_IO = -4,        /* file could not be read / parsed */
+    VFS_ERR_NOMEM = -5      /* allocation failed */
+} vfs_status;
+
+/* Flags for vfs_params.flags. */
+#define VFS_FLAG_TRACK_IDS 0x1u   /* carry a particle-id buffer through every reorder */
+#define VFS_FLAG_NO_GRAPH 0x2u    /* launch substeps eagerly instead of replaying a hipGraph */
+#define VFS_FLAG_STORE_ACCEL 0x4u /* keep the per-particle acceleration buffer up to date */
+
+/*
+ * Parameters of one scene.  Field meaning follows
+ *   SimulationParameters            src/simulation.h:10-13
+ *   SPHModel::Parameters            src/models/model.h:14-21
+ *   WCSPHWithBoundaryModel::Parameters src/models/wcsph_with_boundary_model.h:26-33
+ * (the two misspelled names `expoent` and `viscosity_strenght` are kept).
+ */
+typedef struct vfs_params {
+    float gravity[3];
+    float smooth_radius;
+    float time_scale;
+    int32_t iterations;       /* substeps per vfs_sim_step */
+    int32_t n_particles;
+    float fixed_dt;
+    float target_density;
+    float bbox_size[3];       /* gfx::BoundingBox{size, pos} (src/gfx/mesh.h:13-16) */
+    float bbox_pos[3];
+    float wall_stiffness;
+    float stiffness;
+    float expoent;
+    float viscosity_strenght;
+    float cell_size;          /* SpatialHash cell size; 0 selects smooth_radius (model.cpp:85-86) */
+    uint32_t flags;           /* VFS_FLAG_* */
+} vfs_params;
+
+/*
+ * One boundary object, as WCSPHWithBoundaryModel::BoundaryObjectInfo
+ * (src/models/wcsph_with_boundary_model.h:16-24), with host grids that the
+ * library uploads.  Matrices are column-major 4x4 (glm layout):
+ *   transform = inverse(object matrix)   (src/scenes/generic_scene.cpp:135)
+ *   rotation  = mat4_cast(rotation)      (src/scenes/generic_scene.cpp:136)
+ * Grid index of node (i,j,k) is i + nx*(j + ny*k).
+ */
+typedef struct vfs_boundary_object {
+    float transform[16];
+    float rotation[16];
+    float box_size[3];
+    float box_pos[3];
+    uint32_t resolution[3];
+    uint32_t reserved;
+    const float *sdf_grid;
+    const float *volume_map_grid;
+} vfs_boundary_object;
+
+/* Device addresses of a simulation's particle buffers (SPHModel::GetDataBuffers,
+ * src/models/model.h:38-43).  Positions and velocities are float4 per particle
+ * (x, y, z, unused); densities are float; particle_ids is NULL without
+ * VFS_FLAG_TRACK_IDS; accelerations is NULL without VFS_FLAG_STORE_ACCEL. */
+typedef struct vfs_device_buffers {
+    float *positions;
+    float *velocities;
+    float *densities;
+    float *accelerations;
+    uint32_t *particle_ids;
+    uint32_t *sorted_keys;
+    uint32_t *sorted_indices;
+    uint32_t *cell_start;     /* n_particles + 1 entries: bucket k is [cell_start[k], cell_start[k+1]) */
+    uint32_t n_particles;
+} vfs_device_buffers;
+
+typedef struct vfs_sim vfs_sim;
+typedef struct vfs_spatial_hash vfs_spatial_hash;
+
+/* ---- library ---------------------------------------------------------- */
+const char *vfs_last_error(void);
+int vfs_abi_version(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int vfs_device_count(int *count);
+
+/* ---- device memory and streams (plumbing for callers without their own allocator) */
+int vfs_dev_malloc(void **ptr, size_t bytes);
+int vfs_dev_free(void *ptr);
+int vfs_memcpy_h2d(void *dst_dev, const void *src_host, size_t bytes);
+int vfs_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes);
+int vfs_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes);
+int vfs_memset_d32(void *dst_dev, uint32_t value, size_t count);
+int vfs_stream_create(void **stream);
+int vfs_stream_destroy(void *stream);
+int vfs_stream_synchronize(void *stream);
+
+/* ---- simulation (the substep engine) ----------------------------------- */
+
+/* Replaces WCSPHWithBoundaryModel(params) + Init (wcsph_with_boundary_model.cpp:24-79)
+ * and GenericScene::CreateBoundaryObjectBuffer (generic_scene.cpp:125-145). */
+int vfs_sim_create(const vfs_params *params, const vfs_boundary_object *objects,
+                   uint32_t n_objects, vfs_sim **out);
+/* Replaces SPHModel::Clear / WCSPHWithBoundaryModel buffers Destroy (model.cpp:102-111). */
+int vfs_sim_destroy(vfs_sim *sim);
+/* Replaces SPHModel::SetParticleState (model.cpp:139-147): AoS float3 host arrays,
+ * densities of the range are zeroed.  count < 0 means n_particles - offset. */
+int vfs_sim_set_state(vfs_sim *sim, const float *positions, const float *velocities,
+                      uint32_t offset, int64_t count);
+/* Runtime parameter update (the ImGui sliders: model.cpp:236-264,
+ * wcsph_with_boundary_model.cpp:119-141, simulation.cpp:90-104).  n_particles
+ * and flags must not change. */
+int vfs_sim_set_params(vfs_sim *sim, const vfs_params *params);
+int vfs_sim_get_params(const vfs_sim *sim, vfs_params *params);
+/* Replaces WCSPHWithBoundaryModel::Step (wcsph_with_boundary_model.cpp:81-117):
+ * params.iterations substeps of dt = time_scale*fixed_dt/iterations. */
+int vfs_sim_step(vfs_sim *sim, void *stream);
+/* n substeps at the same dt (one substep is the unit of work of the benchmark). */
+int vfs_sim_substeps(vfs_sim *sim, uint32_t n, void *stream);
+/* Blocks until every substep queued on the simulation stream has finished. */
+int vfs_sim_synchronize(vfs_sim *sim);
+/* Host copies of the current state (any pointer may be NULL).  positions /
+ * velocities are AoS float3; particle_ids requires VFS_FLAG_TRACK_IDS. */
+int vfs_sim_get_state(vfs_sim *sim, float *positions, float *velocities, float *densities,
+                      uint32_t *particle_ids);
+/* Hash tables of the last substep (SpatialHash buffers, spatial_hash.h:23-26):
+ * sorted keys, the permutation applied by that substep's reorder, and
+ * reference-format offsets (first index of key k, n if empty). */
+int vfs_sim_get_hash(vfs_sim *sim, uint32_t *sorted_keys, uint32_t *sorted_indices,
+                     uint32_t *offsets);
+/* Boundary volume psi and its gradient (float3 AoS) of the last substep
+ * (CalculateBoundaryVolume outputs, wcsph_with_boundary_model.slang:102-151). */
+int vfs_sim_get_boundary(vfs_sim *sim, float *boundary_density, float *boundary_gradient);
+int vfs_sim_device_buffers(vfs_sim *sim, vfs_device_buffers *out);
+/* Runs n substeps eagerly (advancing the state) with a HIP event pair around
+ * every kernel on the simulation stream.  For each stage writes the mean time
+ * per substep in ms (summed over its launches), the launches per substep and
+ * the stage name (at most max_stages entries; any output array may be NULL). */
+int vfs_sim_profile_stages(vfs_sim *sim, uint32_t n, float *ms, uint32_t *launches,
+                           const char **names, uint32_t max_stages, uint32_t *n_stages);
+
+/* ---- compute layer drop-ins (reference src/compute headers) ------------ */
+
+/* SpatialHash (src/compute/spatial_hash.h:9-34): Init(n, cell_size), Run(positions),
+ * buffers.  positions_dev is float3 AoS (12-byte stride) or float4 (16-byte
+ * stride) as selected by `stride_floats` (3 or 4). */
+int vfs_spatial_hash_create(uint32_t n, float cell_size, vfs_spatial_hash **out);
+int vfs_spatial_hash_set_cell_size(vfs_spatial_hash *h, float cell_size);
+int vfs_spatial_hash_run(vfs_spatial_hash *h, const float *positions_dev, uint32_t stride_floats,
+                         void *stream);
+int vfs_spatial_hash_buffers(vfs_spatial_hash *h, uint32_t **spatial_keys,
+                             uint32_t **spatial_indices, uint32_t **spatial_offsets);
+int vfs_spatial_hash_destroy(vfs_spatial_hash *h);
+
+/* GPUCountSort::Run (src/compute/sort.cpp:107-143): sorts (items, keys) by key
+ * in place, keys <= max_value.  Stable: equal keys keep their input order.
+ * The caller provides workspace of vfs_sort_workspace_size bytes. */
+int vfs_sort_workspace_size(uint32_t n, uint32_t max_value, size_t *bytes);
+int vfs_sort_pairs(uint32_t *items_dev, uint32_t *keys_dev, uint32_t n, uint32_t max_value,
+                   void *workspace, size_t workspace_bytes, void *stream);
+/* GPUScan::Run (src/compute/sort.cpp:57-86): exclusive prefix sum in place. */
+int vfs_scan_workspace_size(uint32_t n, size_t *bytes);
+int vfs_scan_exclusive(uint32_t *elements_dev, uint32_t n, void *workspace, size_t workspace_bytes,
+                       void *stream);
+/* SpatialOffset::Run (src/compute/sort.cpp:155-176). */
+int vfs_spatial_offsets(const uint32_t *sorted_keys_dev, uint32_t *offsets_dev, uint32_t n,
+                        int init, void *stream);
+/* BufferReorder::Reorder (src/compute/sort.cpp:206-219), one float3 AoS buffer. */
+int vfs_reorder_f32x3(const uint32_t *sorted_indices_dev, const float *src_dev, float *dst_dev,
+                      uint32_t n, void *stream);
+
+/* ---- boundary grids (host precompute, src/util/volume_map.cpp:9-54) --- */
+
+/* Analytic shapes whose signed distance the grid generator evaluates. */
+typedef enum vfs_shape_kind {
+    VFS_SHAPE_BOX = 0,        /* solid box: params = {cx, cy, cz, hx, hy, hz} (centre, half extents) */
+    VFS_SHAPE_SPHERE = 1,     /* solid sphere: params = {cx, cy, cz, r} */
+    VFS_SHAPE_BOX_WALLS = 2   /* box container (solid outside): params = {cx, cy, cz, hx, hy, hz} */
+} vfs_shape_kind;
+
+/* Fills sdf and volume map grids (res[0]*res[1]*res[2] floats each) for an
+ * analytic shape, with the reference's domain (shape AABB grown by
+ * tolerance + margin, mesh_sdf.cpp:28-45) and volume-map integrand
+ * (16^3 Gauss-Legendre, volume_map.cpp:9-54).  box_size/box_pos receive the
+ * grid domain.  Runs on host threads (threads <= 0: hardware concurrency). */
+int vfs_volume_map_generate(int shape_kind, const double *shape_params, const uint32_t *res,
+                            double support_radius, double tolerance, double margin,
+                            float *sdf_out, float *vm_out, float *box_size, float *box_pos,
+                            int threads);
+
+/* ---- scenes (src/util/scene_loader.cpp:113-140, src/scenes/generic_scene.cpp) */
+typedef struct vfs_scene vfs_scene;
+/* Parses a scene JSON (reference schema + optional extension keys) and builds
+ * the simulation with its boundary objects and seeded particles. */
+int vfs_scene_load(const char *json_path, const char *resource_root, vfs_scene **out);
+int vfs_scene_sim(vfs_scene *scene, vfs_sim **sim);
+int vfs_scene_reset(vfs_scene *scene);
+int vfs_scene_step(vfs_scene *scene, void *stream);
+int vfs_scene_destroy(vfs_scene *scene);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VFS_C_H */

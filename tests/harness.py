@@ -1,0 +1,188 @@
+"""Test harness: ctypes binding of the CPU oracle (oracle/liboracle.so) and
+helpers shared by the CPU and GPU tests.  The oracle is the checker only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+
+
+def _load_oracle() -> C.CDLL:
+    if not ORACLE_SO.exists():
+        subprocess.run(["make", "-C", str(ROOT / "oracle")], check=True, capture_output=True)
+    L = C.CDLL(str(ORACLE_SO))
+    vp, u32, fp, up, dp = C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.POINTER(C.c_double)
+    sig = {
+        "orc_hash_cell": (u32, [C.c_int32] * 3),
+        "orc_hash_keys": (None, [fp, u32, C.c_float, u32, up]),
+        "orc_stable_sort": (None, [up, u32, u32, up, up]),
+        "orc_offsets": (None, [up, u32, up]),
+        "orc_exclusive_scan": (None, [up, u32, up]),
+        "orc_W_f32": (C.c_float, [vp, C.c_float]),
+        "orc_dW_f32": (C.c_float, [vp, C.c_float]),
+        "orc_W_f64": (C.c_double, [vp, C.c_double]),
+        "orc_dW_f64": (C.c_double, [vp, C.c_double]),
+        "orc_eos_f32": (C.c_float, [vp, C.c_float]),
+        "orc_uniforms_f32": (None, [vp, fp]),
+        "orc_sample_grid_f32": (C.c_float, [fp, up, fp]),
+        "orc_sample_grid_f64": (C.c_double, [fp, up, dp]),
+        "orc_sim_create": (vp, [vp, vp, u32, C.c_int]),
+        "orc_sim_destroy": (None, [vp]),
+        "orc_sim_set_threads": (None, [vp, C.c_int]),
+        "orc_sim_set_state": (None, [vp, fp, fp, u32, u32]),
+        "orc_sim_substeps": (None, [vp, u32]),
+        "orc_sim_get_state": (None, [vp, dp, dp, dp, up, dp]),
+        "orc_sim_get_state_f32": (None, [vp, fp, fp, fp]),
+        "orc_sim_get_hash": (None, [vp, up, up, up]),
+        "orc_sim_get_boundary": (None, [vp, dp, dp]),
+        "orc_gauss_legendre": (None, [C.c_int, dp, dp]),
+        "orc_volume_map": (C.c_int, [C.c_int, dp, up, C.c_double, C.c_double, C.c_double, fp, fp, fp, fp, C.c_int]),
+        "orc_max_threads": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+ORC = _load_oracle()
+
+
+def fp(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def up(a):
+    assert a.dtype == np.uint32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def threads() -> int:
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except Exception:
+        return 4
+
+
+class OracleSim:
+    """CPU restatement of one scene (FP32 parity oracle, or the FP64 twin)."""
+
+    def __init__(self, params, grids=(), fp64: bool = False, nthreads: int | None = None):
+        from vkfluidsim_amd._lib import BoundaryObject
+        self.n = int(params.n_particles)
+        self._grids = list(grids)
+        arr = (BoundaryObject * max(1, len(self._grids)))()
+        for k, g in enumerate(self._grids):
+            arr[k] = g.as_struct()
+        self.h = ORC.orc_sim_create(C.addressof(params), C.addressof(arr), len(self._grids), 1 if fp64 else 0)
+        assert self.h
+        ORC.orc_sim_set_threads(self.h, nthreads or threads())
+
+    def set_state(self, x, v=None):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1)
+        v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, np.float32).reshape(-1)
+        ORC.orc_sim_set_state(self.h, fp(x), fp(v), 0, x.size // 3)
+
+    def substeps(self, k: int):
+        ORC.orc_sim_substeps(self.h, int(k))
+
+    def get_state(self):
+        n = self.n
+        x = np.empty(3 * n)
+        v = np.empty(3 * n)
+        rho = np.empty(n)
+        pid = np.empty(n, np.uint32)
+        a = np.empty(3 * n)
+        ORC.orc_sim_get_state(self.h, dp(x), dp(v), dp(rho), up(pid), dp(a))
+        return x.reshape(n, 3), v.reshape(n, 3), rho, pid, a.reshape(n, 3)
+
+    def get_hash(self):
+        n = self.n
+        k = np.empty(n, np.uint32)
+        p = np.empty(n, np.uint32)
+        o = np.empty(n, np.uint32)
+        ORC.orc_sim_get_hash(self.h, up(k), up(p), up(o))
+        return k, p, o
+
+    def get_boundary(self):
+        n = self.n
+        psi = np.empty(n)
+        g = np.empty(3 * n)
+        ORC.orc_sim_get_boundary(self.h, dp(psi), dp(g))
+        return psi, g.reshape(n, 3)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ORC.orc_sim_destroy(self.h)
+            self.h = None
+
+
+def oracle_volume_map(kind, shape_params, resolution, support_radius, tolerance=0.0, margin=None, nthreads=None):
+    from vkfluidsim_amd import BoundaryGrid
+    if margin is None:
+        margin = 8.0 * support_radius
+    res = np.ascontiguousarray(np.asarray(resolution, np.uint32))
+    sp = np.zeros(6)
+    sp[: len(shape_params)] = shape_params
+    cnt = int(np.prod(res.astype(np.int64)))
+    sdf = np.empty(cnt, np.float32)
+    vm = np.empty(cnt, np.float32)
+    bs = np.empty(3, np.float32)
+    bpos = np.empty(3, np.float32)
+    rc = ORC.orc_volume_map(kind, dp(sp), up(res), support_radius, tolerance, margin, fp(sdf), fp(vm), fp(bs),
+                            fp(bpos), nthreads or threads())
+    assert rc == 0
+    return BoundaryGrid(sdf, vm, tuple(int(r) for r in res), tuple(float(v) for v in bs),
+                        tuple(float(v) for v in bpos))
+
+
+def oracle_keys(x, cell, table):
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    n = x.size // 3
+    keys = np.empty(n, np.uint32)
+    ORC.orc_hash_keys(fp(x), n, cell, table, up(keys))
+    return keys
+
+
+def oracle_sort(keys, max_value):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    n = keys.size
+    sk = np.empty(n, np.uint32)
+    perm = np.empty(n, np.uint32)
+    ORC.orc_stable_sort(up(keys), n, max_value, up(sk), up(perm))
+    return sk, perm
+
+
+def oracle_offsets(sorted_keys):
+    sorted_keys = np.ascontiguousarray(sorted_keys, np.uint32)
+    n = sorted_keys.size
+    o = np.empty(n, np.uint32)
+    ORC.orc_offsets(up(sorted_keys), n, up(o))
+    return o
+
+
+def rel_err(a, b) -> float:
+    """max |a - b| over the field, relative to max |b| (the field scale)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    scale = max(float(np.max(np.abs(b))) if b.size else 0.0, 1e-30)
+    return float(np.max(np.abs(a - b))) / scale if a.size else 0.0
+
+
+def by_id(arr, pid):
+    out = np.empty_like(arr)
+    out[pid] = arr
+    return out

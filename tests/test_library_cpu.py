@@ -1,0 +1,74 @@
+"""CPU-tier checks of the C ABI library: it loads, exports every declared
+symbol, and rejects bad arguments before touching a device."""
+import ctypes as C
+from pathlib import Path
+
+import pytest
+
+import vkfluidsim_amd as V
+from vkfluidsim_amd import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+SCENES = ROOT / "tests" / "golden" / "scenes"
+
+
+def test_every_declared_symbol_is_exported():
+    names = _lib.symbols_declared(ROOT / "include" / "vfs_c.h")
+    assert len(names) >= 40
+    missing = [n for n in names if not _lib.exported(n)]
+    assert missing == []
+    # and the binding covers all of them
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_abi_version_and_device_count():
+    assert V.lib.vfs_abi_version() == 1
+    assert V.device_count() >= 0
+
+
+def _err(fn, *args):
+    rc = fn(*args)
+    return rc, V.lib.vfs_last_error().decode()
+
+
+def test_sim_create_rejects_bad_parameters():
+    out = C.c_void_p()
+    rc, msg = _err(V.lib.vfs_sim_create, None, None, 0, C.byref(out))
+    assert rc == -1 and "null" in msg
+    p = V.make_params(10, 0.04, dt=1e-4)
+    p.iterations = 0
+    rc, msg = _err(V.lib.vfs_sim_create, C.byref(p), None, 0, C.byref(out))
+    assert rc == -1 and "iterations" in msg
+    p = V.make_params(-5, 0.04, dt=1e-4)
+    rc, msg = _err(V.lib.vfs_sim_create, C.byref(p), None, 0, C.byref(out))
+    assert rc == -1 and "n_particles" in msg
+    p = V.make_params(10, 0.04, dt=1e-4, cell_size=0.02)
+    rc, msg = _err(V.lib.vfs_sim_create, C.byref(p), None, 0, C.byref(out))
+    assert rc == -1 and "cell_size" in msg
+    p = V.make_params(10, 0.04, dt=1e-4)
+    rc, msg = _err(V.lib.vfs_sim_create, C.byref(p), None, 1, C.byref(out))
+    assert rc == -1 and "objects" in msg
+    o = V.BoundaryObject()
+    rc, msg = _err(V.lib.vfs_sim_create, C.byref(p), C.byref(o), 1, C.byref(out))
+    assert rc == -1 and "grid" in msg
+
+
+def test_sizes_and_workspaces():
+    ws = C.c_size_t(0)
+    assert V.lib.vfs_sort_workspace_size(1 << 30, 5, C.byref(ws)) == -1
+    assert V.lib.vfs_sort_workspace_size(1000, 999, C.byref(ws)) == 0 and ws.value > 8000
+    assert V.lib.vfs_scan_workspace_size(1000, C.byref(ws)) == 0 and ws.value > 0
+    h = C.c_void_p()
+    assert V.lib.vfs_spatial_hash_create(10, 0.0, C.byref(h)) == -1
+    assert V.lib.vfs_sim_substeps(None, 1, None) == -1
+
+
+@pytest.mark.parametrize("name,code,needle", [("malformed.json", -4, "parse error"),
+                                              ("missing_key.json", -4, "outerWallStiffness"),
+                                              ("does_not_exist.json", -4, "cannot open"),
+                                              ("mesh_objects.json", -1, "OBJ mesh")])
+def test_scene_loader_errors(name, code, needle):
+    out = C.c_void_p()
+    rc, msg = _err(V.lib.vfs_scene_load, str(SCENES / name).encode(), b"", C.byref(out))
+    assert rc == code and needle in msg
+    assert out.value is None

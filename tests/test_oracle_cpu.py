@@ -1,0 +1,393 @@
+"""Pins of the CPU oracle (no GPU).  The reference has no tests or golden data,
+so each part of the oracle is pinned by known answers computed from the closed
+forms, by textbook properties, by brute force on small inputs, or by an
+independent library routine (numpy/scipy), chosen so that a dropped term,
+wrong sign, wrong index or transposed operand fails."""
+import math
+
+import numpy as np
+import pytest
+from numpy.polynomial import legendre
+
+import harness as H
+from harness import ORC
+
+from vkfluidsim_amd import make_params
+from vkfluidsim_amd import configs as CF
+
+
+# ------------------------------------------------------------------ hashing
+# HashCell3D known answers (closed form evaluated independently in the survey).
+KAT = [((0, 0, 0), 63775), ((1, 2, 3), 71376), ((-1, 0, 0), 63774), ((-1, -1, -1), 61224),
+       ((-26, 0, 0), 1975734194), ((25, 25, 25), 450570913), ((40, 20, 40), 440873345),
+       ((99, 49, 49), 450647960)]
+KAT_MOD = {32768: [31007, 5840, 31006, 28456, 20402, 10913, 12673, 22424],
+           1048576: [63775, 71376, 63774, 61224, 217010, 731809, 471425, 808856],
+           8388608: [63775, 71376, 63774, 61224, 4411314, 5974689, 4665729, 6051736]}
+
+
+def test_hash_cell_known_answers():
+    for cell, h in KAT:
+        assert ORC.orc_hash_cell(*cell) == h
+
+
+def test_hash_cell_local_strides_and_block_primes():
+    # inside a 50-block: +1 per x cell, +50 per y cell, +2500 per z cell
+    for c in [(0, 0, 0), (-20, 3, 7), (10, -24, 20)]:
+        h0 = ORC.orc_hash_cell(*c)
+        assert ORC.orc_hash_cell(c[0] + 1, c[1], c[2]) == h0 + 1
+        assert ORC.orc_hash_cell(c[0], c[1] + 1, c[2]) == h0 + 50
+        assert ORC.orc_hash_cell(c[0], c[1], c[2] + 1) == h0 + 2500
+    # block (1,0,0) starts at local (0, .., ..) plus prime 15823
+    base = ORC.orc_hash_cell(-25, -25, -25)
+    assert base == 0
+    assert ORC.orc_hash_cell(25, -25, -25) == 15823
+    assert ORC.orc_hash_cell(-25, 25, -25) == 9737333
+    assert ORC.orc_hash_cell(-25, -25, 25) == 440817757
+
+
+@pytest.mark.parametrize("table", sorted(KAT_MOD))
+def test_keys_mod_table(table):
+    for (cell, _), want in zip(KAT, KAT_MOD[table]):
+        # a position strictly inside the cell, cell size 0.25
+        x = (np.array(cell, np.float64) + 0.5) * 0.25
+        assert H.oracle_keys(x.astype(np.float32), 0.25, table)[0] == want
+
+
+def test_keys_floor_not_truncation():
+    # x = -0.01 with cell 0.1 lies in cell -1 (floor), not 0 (truncation)
+    k = H.oracle_keys(np.array([-0.01, 0.05, 0.05], np.float32), 0.1, 1 << 20)[0]
+    assert k == 63774
+
+
+# ------------------------------------------------------------- sort & scan
+@pytest.mark.parametrize("n", [1, 2, 17, 1000, 65536])
+def test_stable_sort_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, n, n).astype(np.uint32)
+    sk, perm = H.oracle_sort(keys, n - 1)
+    ref = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(perm, ref)
+    np.testing.assert_array_equal(sk, keys[ref])
+
+
+def test_offsets_brute_force():
+    rng = np.random.default_rng(1)
+    n = 5000
+    sk = np.sort(rng.integers(0, n, n).astype(np.uint32))
+    off = H.oracle_offsets(sk)
+    ref = np.full(n, n, np.uint32)
+    u, first = np.unique(sk, return_index=True)
+    ref[u] = first
+    np.testing.assert_array_equal(off, ref)
+
+
+def test_exclusive_scan():
+    rng = np.random.default_rng(2)
+    a = rng.integers(0, 1000, 777).astype(np.uint32)
+    out = np.empty_like(a)
+    ORC.orc_exclusive_scan(H.up(a), a.size, H.up(out))
+    np.testing.assert_array_equal(out, np.concatenate([[0], np.cumsum(a)[:-1]]).astype(np.uint32))
+
+
+# ------------------------------------------------------------------ kernels
+def _p(h, **kw):
+    return make_params(1, h, dt=1e-4, **kw)
+
+
+@pytest.mark.parametrize("h", [0.04, 0.02, 0.2])
+def test_uniform_constants_known_answers(h):
+    out = np.empty(4, np.float32)
+    p = _p(h)
+    ORC.orc_uniforms_f32(_addr(p), H.fp(out))
+    mass, sigma, grad_sigma, dt = out
+    want = {0.04: (39788.73, 1.49208e8, 6.4e-3), 0.02: (318309.9, 4.77465e9, 8.0e-4),
+            0.2: (318.3099, 47746.48, 0.8)}[h]
+    assert sigma == pytest.approx(want[0], rel=2e-6)
+    assert grad_sigma == pytest.approx(want[1], rel=2e-5)
+    assert mass == pytest.approx(want[2], rel=1e-6)
+    assert dt == pytest.approx(1e-4, rel=1e-7)
+
+
+def _addr(p):
+    import ctypes
+    return ctypes.addressof(p)
+
+
+def textbook_w(r, h):
+    q = r / h
+    s = 8.0 / (math.pi * h ** 3)
+    if q <= 0.5:
+        return s * (6 * q ** 3 - 6 * q ** 2 + 1)
+    if q <= 1.0:
+        return 2 * s * (1 - q) ** 3
+    return 0.0
+
+
+@pytest.mark.parametrize("h", [0.04, 0.2])
+def test_cubic_spline_textbook_values(h):
+    p = _p(h)
+    for q in [0.0, 0.1, 0.3, 0.5, 0.5001, 0.75, 0.99, 1.0, 1.2]:
+        r = q * h
+        w32 = ORC.orc_W_f32(_addr(p), np.float32(r))
+        w64 = ORC.orc_W_f64(_addr(p), r)
+        ref = textbook_w(r, h)
+        assert w64 == pytest.approx(ref, rel=1e-6, abs=1e-9 * textbook_w(0, h))
+        assert w32 == pytest.approx(ref, rel=1e-5, abs=1e-6 * textbook_w(0, h))
+
+
+def test_cubic_spline_normalised():
+    h = 0.04
+    p = _p(h)
+    from scipy.integrate import quad
+    val, _ = quad(lambda r: 4 * math.pi * r * r * ORC.orc_W_f64(_addr(p), r), 0, h, points=[h / 2], limit=200)
+    assert val == pytest.approx(1.0, rel=1e-6)
+
+
+def test_grad_kernel_carries_extra_inverse_h():
+    """The shader gradient is (1/h) * dW/dr (kernels_3d.slang:80-81)."""
+    h = 0.04
+    p = _p(h)
+    for q in [0.2, 0.4, 0.6, 0.8, 0.95]:
+        r = q * h
+        e = 1e-7
+        fd = (ORC.orc_W_f64(_addr(p), r + e) - ORC.orc_W_f64(_addr(p), r - e)) / (2 * e)
+        assert ORC.orc_dW_f64(_addr(p), r) == pytest.approx(fd / h, rel=1e-6)
+        assert fd < 0  # W decreases
+    assert ORC.orc_dW_f64(_addr(p), 0.0) == 0.0
+    assert ORC.orc_dW_f64(_addr(p), 1.01 * h) == 0.0
+
+
+@pytest.mark.parametrize("gamma", [7.0, 1.0, 3.5])
+def test_eos_closed_form(gamma):
+    p = make_params(1, 0.04, dt=1e-4, rho0=1000.0, stiffness=1000.0, gamma=gamma)
+    for rho in [500.0, 999.9, 1000.0, 1001.0, 1050.0, 1300.0]:
+        got = ORC.orc_eos_f32(_addr(p), np.float32(rho))
+        # the shader divides in float: take the FP32 ratio, then the exact power
+        ratio = float(np.float32(max(rho, 1000.0)) / np.float32(1000.0))
+        ref = 1000.0 * (ratio ** gamma - 1.0)
+        assert got == pytest.approx(ref, rel=1e-6, abs=2e-4)
+
+
+# ------------------------------------------------------------------ sampler
+def test_sample_grid_reproduces_linear_fields():
+    rng = np.random.default_rng(4)
+    n = np.array([7, 5, 9], np.uint32)
+    c0, c = 0.3, np.array([1.5, -2.0, 0.75])
+    ii, jj, kk = np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+    # grid index i + nx*(j + ny*k): flatten in (k, j, i) order
+    u = np.stack([ii / (n[0] - 1), jj / (n[1] - 1), kk / (n[2] - 1)], -1)
+    f = c0 + u @ c
+    grid = np.ascontiguousarray(np.transpose(f, (2, 1, 0)).reshape(-1), np.float32)
+    for _ in range(200):
+        uvw = rng.uniform(0, 1, 3)
+        got = ORC.orc_sample_grid_f64(H.fp(grid), H.up(n), H.dp(uvw.astype(np.float64)))
+        assert got == pytest.approx(c0 + uvw @ c, abs=2e-6)
+    # nodes, including the upper faces (uvw == 1) where the base cell is clamped
+    for idx in [(0, 0, 0), (6, 4, 8), (3, 2, 5), (6, 0, 8)]:
+        uvw = np.array(idx, np.float64) / (n - 1)
+        want = grid[idx[0] + n[0] * (idx[1] + n[1] * idx[2])]
+        assert ORC.orc_sample_grid_f64(H.fp(grid), H.up(n), H.dp(uvw)) == pytest.approx(float(want), abs=1e-6)
+
+
+# ------------------------------------------------------------- full steps
+def _scene(x, h, g=(0.0, 0.0, 0.0), nu=0.0, box=(100.0, 100.0, 100.0), pos=(-50.0, -50.0, -50.0), fp64=False,
+           dt=1e-5, v=None):
+    p = make_params(x.shape[0], h, dt=dt, gravity=g, viscosity=nu, box_size=box, box_pos=pos, flags=1)
+    o = H.OracleSim(p, [], fp64=fp64, nthreads=2)
+    o.set_state(x, v)
+    return o, p
+
+
+def _cloud(n, seed, extent=0.2):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0, extent, (n, 3)).astype(np.float32)
+
+
+def test_density_matches_brute_force():
+    h = 0.04
+    x = _cloud(2000, 3)
+    o, p = _scene(x, h)
+    o.substeps(1)
+    xs, _, rho, pid, _ = o.get_state()
+    # brute force on the (reordered) positions before integration: recover them from ids
+    x0 = x.astype(np.float64)[pid]
+    d = np.linalg.norm(x0[:, None, :] - x0[None, :, :], axis=-1)
+    out = np.zeros_like(rho)
+    mass = 1000.0 * 0.8 * h ** 3 / 8
+    for i in range(len(rho)):
+        r = d[i][d[i] <= h]
+        out[i] = mass * sum(textbook_w(rr, h) for rr in r)
+    assert np.max(np.abs(rho - out) / out) < 2e-5
+
+
+def test_pressure_conserves_momentum_without_boundaries():
+    h = 0.04
+    x = _cloud(1500, 5, extent=0.12)  # dense: rho > rho0 so that pressure acts
+    o, p = _scene(x, h)
+    o.substeps(1)
+    _, _, rho, _, a = o.get_state()
+    assert rho.max() > 1000.0
+    tot = a.sum(axis=0)
+    assert np.all(np.abs(tot) <= 1e-4 * np.abs(a).sum(axis=0))
+
+
+def test_viscosity_vanishes_for_rigid_translation():
+    h = 0.04
+    x = _cloud(800, 6, extent=0.15)
+    v = np.tile(np.array([[0.3, -0.2, 0.1]], np.float32), (x.shape[0], 1))
+    a1, _ = _scene(x, h, nu=0.0, v=v)
+    a2, _ = _scene(x, h, nu=0.5, v=v)
+    a1.substeps(1)
+    a2.substeps(1)
+    np.testing.assert_array_equal(a1.get_state()[4], a2.get_state()[4])
+
+
+def test_free_fall_closed_form():
+    """A lone particle (density = self term < rho0, no pressure) falls freely."""
+    x = np.array([[0.5, 0.5, 0.5]], np.float32)
+    g, dt, n = (0.0, -9.81, 0.0), 1e-3, 50
+    o, _ = _scene(x, 0.04, g=g, dt=dt, fp64=True)
+    o.substeps(n)
+    xs, vs, _, _, _ = o.get_state()
+    assert vs[0, 1] == pytest.approx(-9.81 * dt * n, rel=1e-6)
+    assert xs[0, 1] == pytest.approx(0.5 - 9.81 * dt * dt * n * (n + 1) / 2, rel=1e-6)
+
+
+def test_wall_penalty_closed_form():
+    x = np.array([[-0.05, 0.5, 0.5], [0.5, 1.2, 0.5]], np.float32)
+    o, _ = _scene(x, 0.04, box=(1.0, 1.0, 1.0), pos=(0.0, 0.0, 0.0), dt=1e-3, fp64=True)
+    o.substeps(1)
+    _, v, _, pid, a = o.get_state()
+    v = H.by_id(v, pid)
+    # a = -k * diff, diff = min(x - lo, 0) - min(hi - x, 0)
+    assert v[0, 0] == pytest.approx(-1e4 * (-0.05) * 1e-3, rel=1e-6)
+    assert v[1, 1] == pytest.approx(-1e4 * (0.2) * 1e-3, rel=1e-5)
+
+
+def test_key_collisions_double_count_like_the_reference():
+    """With a 7-entry table the 27 neighbour keys repeat; each repetition visits
+    the bucket again.  Brute force over the repeated key list."""
+    x = np.array([[0.10, 0.10, 0.10], [0.12, 0.11, 0.10], [0.15, 0.10, 0.12], [0.30, 0.30, 0.30],
+                  [0.31, 0.33, 0.29], [0.11, 0.13, 0.14], [0.50, 0.10, 0.10]], np.float32)
+    h = 0.05
+    o, _ = _scene(x, h)
+    o.substeps(1)
+    _, _, rho, pid, _ = o.get_state()
+    n = 7
+    keys = H.oracle_keys(x, h, n)
+    mass = 1000.0 * 0.8 * h ** 3 / 8
+    want = np.zeros(n)
+    for i in range(n):
+        c = np.floor(x[i].astype(np.float32) / np.float32(h)).astype(int)
+        total = 0.0
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    k = ORC.orc_hash_cell(int(c[0] + dx), int(c[1] + dy), int(c[2] + dz)) % n
+                    for j in np.nonzero(keys == k)[0]:
+                        r = float(np.linalg.norm(x[i].astype(np.float64) - x[j]))
+                        if r <= h:
+                            total += mass * textbook_w(r, h)
+        want[i] = total
+    got = H.by_id(rho, pid)
+    np.testing.assert_allclose(got, want, rtol=2e-5)
+    assert np.any(want > mass * textbook_w(0, h) * 1.5)  # some buckets were counted twice
+
+
+def test_fp32_and_fp64_twins_agree_on_one_substep():
+    w = CF.dam_break(side=(10, 10, 10), map_res=20)
+    g = H.oracle_volume_map(*w.shapes[0][:2], w.shapes[0][2], w.h, nthreads=4)
+    p = make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, flags=1)
+    a = H.OracleSim(p, [g], nthreads=4)
+    b = H.OracleSim(p, [g], fp64=True, nthreads=4)
+    a.set_state(w.x)
+    b.set_state(w.x)
+    a.substeps(1)
+    b.substeps(1)
+    for fa, fb in zip(a.get_state()[:3], b.get_state()[:3]):
+        assert H.rel_err(fa, fb) < 1e-5
+
+
+# -------------------------------------------------------- volume maps
+def test_gauss_legendre_nodes_match_numpy():
+    x = np.empty(16)
+    w = np.empty(16)
+    ORC.orc_gauss_legendre(16, H.dp(x), H.dp(w))
+    xr, wr = legendre.leggauss(16)
+    np.testing.assert_allclose(x, xr, atol=1e-14)
+    np.testing.assert_allclose(w, wr, atol=1e-14)
+    assert x[-1] == pytest.approx(0.989400934991649932596, abs=1e-15)  # Abramowitz & Stegun 25.4
+
+
+def _vm_numpy(node, sdf, h):
+    """Independent evaluation of one volume-map node (volume_map.cpp:9-54)."""
+    xg, wg = legendre.leggauss(16)
+    pts = np.stack(np.meshgrid(xg * h, xg * h, xg * h, indexing="ij"), -1).reshape(-1, 3)
+    wts = np.einsum("i,j,k->ijk", wg, wg, wg).reshape(-1) * h ** 3
+    inside_ball = np.sum(pts ** 2, axis=1) <= h * h
+    d = sdf(node[None, :] + pts)
+    wz = textbook_w(0.0, h)
+    gam = np.where(d <= 0, 1.0, np.where(d < h, [textbook_w(v, h) / wz if v < h else 0 for v in d], 0.0))
+    return 0.8 * float(np.sum(wts * gam * inside_ball))
+
+
+def test_volume_map_box_walls_nodes():
+    h = 0.04
+    res = (12, 10, 10)
+    centre, half = np.array([0.8, 0.4, 0.4]), np.array([0.8, 0.4, 0.4])
+    g = H.oracle_volume_map(2, (*centre, *half), res, h, nthreads=4)
+
+    def sdf(q):
+        qq = np.abs(q - centre) - half
+        out = np.linalg.norm(np.maximum(qq, 0), axis=-1) + np.minimum(np.max(qq, axis=-1), 0)
+        return -out
+
+    lo = np.array(g.box_pos, np.float64)
+    step = (np.array(g.box_size, np.float64)) / (np.array(res) - 1)
+    np.testing.assert_allclose(g.box_pos, centre - half - 8 * h, rtol=1e-6)
+    rng = np.random.default_rng(0)
+    checked = 0
+    for _ in range(40):
+        i, j, k = (int(rng.integers(0, r)) for r in res)
+        node = lo + step * np.array([i, j, k])
+        idx = i + res[0] * (j + res[1] * k)
+        assert g.sdf[idx] == pytest.approx(float(sdf(node[None])[0]), abs=1e-5)
+        want = _vm_numpy(node, sdf, h) if sdf(node[None])[0] <= 2 * h else 0.0
+        assert g.vm[idx] == pytest.approx(want, rel=1e-4, abs=1e-9)
+        checked += want > 0
+    assert checked > 5
+
+
+def test_volume_map_half_space_against_1d_integral():
+    """Far from edges the walls are a half-space: VM(d) = 0.8 * int_{-h}^{h}
+    gamma(d + s) * pi (h^2 - s^2) ds.  The reference's 16^3 rule over the cube
+    integrates the ball indicator with a fixed bias (deep inside the solid it
+    returns Q_ball instead of 0.8 * 4/3 pi h^3); after removing that factor the
+    map follows the exact 1-D integral to 1% of the full-ball value."""
+    from scipy.integrate import quad
+    h = 0.04
+    g = H.oracle_volume_map(2, (0.5, 0.5, 0.5, 0.5, 0.5, 0.5), (41, 41, 41), h, nthreads=4)
+    lo = np.array(g.box_pos, np.float64)
+    step = np.array(g.box_size, np.float64) / 40
+    wz = textbook_w(0, h)
+    full = 0.8 * 4.0 / 3.0 * math.pi * h ** 3
+
+    def gamma(d):
+        return 1.0 if d <= 0 else (textbook_w(d, h) / wz if d < h else 0.0)
+
+    deep = g.vm[0 + 41 * (20 + 41 * 20)]  # node 8h outside the wall: whole ball in the solid
+    bias = deep / full
+    assert 1.0 < bias < 1.04
+    seen = 0
+    for i in range(41):
+        node = lo + step * np.array([i, 20, 20])
+        d = node[0]  # signed distance to the wall x = 0 (positive inside the container)
+        if not (-2 * h < d < 2 * h):
+            continue
+        ref = 0.8 * quad(lambda s: gamma(d + s) * math.pi * (h * h - s * s), -h, h, points=[-d, h - d],
+                         limit=200)[0]
+        got = g.vm[i + 41 * (20 + 41 * 20)]
+        assert abs(got - bias * ref) <= 0.01 * full
+        seen += 1
+    assert seen >= 3

@@ -1,0 +1,310 @@
+"""vkfluidsim_amd: MI355X (gfx950) WCSPH substep engine behind a C ABI.
+
+Python mirror of the reference model interface (luihabl/VkFluidSim
+src/models/model.h:12-114, src/models/wcsph_with_boundary_model.h:14-49) over
+libvfs.so.  Every method calls the HIP library; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BoundaryObject, DeviceArray, Params, VfsError, VFS_FLAG_NO_GRAPH, VFS_FLAG_STORE_ACCEL,
+                   VFS_FLAG_TRACK_IDS, VFS_SHAPE_BOX, VFS_SHAPE_BOX_WALLS, VFS_SHAPE_SPHERE, call, fptr, lib, uptr)
+
+__all__ = ["Sim", "Params", "BoundaryGrid", "SpatialHash", "VfsError", "make_params", "volume_map",
+           "sort_pairs", "scan_exclusive", "spatial_offsets", "reorder_f32x3", "Scene", "lib",
+           "VFS_FLAG_TRACK_IDS", "VFS_FLAG_NO_GRAPH", "VFS_FLAG_STORE_ACCEL",
+           "VFS_SHAPE_BOX", "VFS_SHAPE_SPHERE", "VFS_SHAPE_BOX_WALLS", "device_count"]
+
+device_count = _lib.device_count
+
+IDENTITY16 = np.eye(4, dtype=np.float32).reshape(-1)
+
+
+def make_params(n: int, h: float, *, dt: float, box_size=(1.0, 1.0, 1.0), box_pos=(0.0, 0.0, 0.0),
+                rho0: float = 1000.0, stiffness: float = 1000.0, gamma: float = 7.0, viscosity: float = 0.01,
+                gravity=(0.0, -9.81, 0.0), wall_stiffness: float = 1e4, iterations: int = 1,
+                time_scale: float = 1.0, cell_size: float = 0.0, flags: int = 0) -> Params:
+    p = Params()
+    p.gravity[:] = [float(g) for g in gravity]
+    p.smooth_radius = h
+    p.time_scale = time_scale
+    p.iterations = iterations
+    p.n_particles = n
+    p.fixed_dt = dt
+    p.target_density = rho0
+    p.bbox_size[:] = [float(s) for s in box_size]
+    p.bbox_pos[:] = [float(s) for s in box_pos]
+    p.wall_stiffness = wall_stiffness
+    p.stiffness = stiffness
+    p.expoent = gamma
+    p.viscosity_strenght = viscosity
+    p.cell_size = cell_size
+    p.flags = flags
+    return p
+
+
+@dataclass
+class BoundaryGrid:
+    """One boundary object: SDF + volume-map grids and its transforms."""
+    sdf: np.ndarray
+    vm: np.ndarray
+    resolution: tuple
+    box_size: tuple
+    box_pos: tuple
+    transform: np.ndarray = field(default_factory=lambda: IDENTITY16.copy())
+    rotation: np.ndarray = field(default_factory=lambda: IDENTITY16.copy())
+
+    def as_struct(self) -> BoundaryObject:
+        o = BoundaryObject()
+        o.transform[:] = [float(v) for v in np.asarray(self.transform, np.float32)]
+        o.rotation[:] = [float(v) for v in np.asarray(self.rotation, np.float32)]
+        o.box_size[:] = [float(v) for v in self.box_size]
+        o.box_pos[:] = [float(v) for v in self.box_pos]
+        o.resolution[:] = [int(v) for v in self.resolution]
+        o.sdf_grid = fptr(self.sdf)
+        o.volume_map_grid = fptr(self.vm)
+        return o
+
+
+def volume_map(kind: int, shape_params, resolution, support_radius: float, *, tolerance: float = 0.0,
+               margin: float | None = None) -> BoundaryGrid:
+    """SDF + volume map of an analytic shape on the GPU (vfs_volume_map_generate)."""
+    if margin is None:
+        margin = 8.0 * support_radius
+    res = np.ascontiguousarray(np.asarray(resolution, dtype=np.uint32))
+    sp = np.zeros(6, dtype=np.float64)
+    sp[: len(shape_params)] = shape_params
+    cnt = int(np.prod(res.astype(np.int64)))
+    sdf = np.empty(cnt, np.float32)
+    vm = np.empty(cnt, np.float32)
+    bs = np.empty(3, np.float32)
+    bp = np.empty(3, np.float32)
+    call("vfs_volume_map_generate", kind, sp.ctypes.data_as(C.POINTER(C.c_double)), uptr(res),
+         support_radius, tolerance, margin, fptr(sdf), fptr(vm), fptr(bs), fptr(bp), 0)
+    return BoundaryGrid(sdf, vm, tuple(int(r) for r in res), tuple(float(v) for v in bs),
+                        tuple(float(v) for v in bp))
+
+
+class Sim:
+    """WCSPHWithBoundaryModel through the C ABI (one scene, one GPU)."""
+
+    def __init__(self, params: Params, objects: list[BoundaryGrid] | None = None):
+        self.params = params
+        self.n = int(params.n_particles)
+        self._objects = list(objects or [])
+        arr = (BoundaryObject * max(1, len(self._objects)))()
+        for k, o in enumerate(self._objects):
+            arr[k] = o.as_struct()
+        h = C.c_void_p()
+        call("vfs_sim_create", C.byref(params), arr if self._objects else None, len(self._objects), C.byref(h))
+        self.handle = h.value
+
+    # SetParticleState (model.cpp:139-147)
+    def set_state(self, x: np.ndarray, v: np.ndarray | None = None, offset: int = 0, count: int = -1) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+        v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1)
+        if count < 0:
+            count = x.size // 3
+        call("vfs_sim_set_state", self.handle, fptr(x), fptr(v), offset, count)
+
+    def set_params(self, params: Params) -> None:
+        call("vfs_sim_set_params", self.handle, C.byref(params))
+        self.params = params
+
+    def step(self, stream=None) -> None:
+        call("vfs_sim_step", self.handle, stream)
+
+    def substeps(self, k: int, stream=None) -> None:
+        call("vfs_sim_substeps", self.handle, int(k), stream)
+
+    def synchronize(self) -> None:
+        call("vfs_sim_synchronize", self.handle)
+
+    def get_state(self, ids: bool = False):
+        n = self.n
+        x = np.empty(3 * n, np.float32)
+        v = np.empty(3 * n, np.float32)
+        rho = np.empty(n, np.float32)
+        pid = np.empty(n, np.uint32) if ids else None
+        call("vfs_sim_get_state", self.handle, fptr(x), fptr(v), fptr(rho), uptr(pid) if ids else None)
+        return x.reshape(n, 3), v.reshape(n, 3), rho, pid
+
+    def get_hash(self):
+        n = self.n
+        k = np.empty(n, np.uint32)
+        p = np.empty(n, np.uint32)
+        o = np.empty(n, np.uint32)
+        call("vfs_sim_get_hash", self.handle, uptr(k), uptr(p), uptr(o))
+        return k, p, o
+
+    def get_boundary(self):
+        n = self.n
+        psi = np.empty(n, np.float32)
+        g = np.empty(3 * n, np.float32)
+        call("vfs_sim_get_boundary", self.handle, fptr(psi), fptr(g))
+        return psi, g.reshape(n, 3)
+
+    def device_buffers(self) -> _lib.DeviceBuffers:
+        b = _lib.DeviceBuffers()
+        call("vfs_sim_device_buffers", self.handle, C.byref(b))
+        return b
+
+    def profile(self, k: int):
+        """[(stage, ms per substep, launches per substep)] over k eager substeps."""
+        m = 16
+        ms = (C.c_float * m)()
+        la = (C.c_uint32 * m)()
+        names = (C.c_char_p * m)()
+        cnt = C.c_uint32(0)
+        call("vfs_sim_profile_stages", self.handle, int(k), ms, la, names, m, C.byref(cnt))
+        return [(names[i].decode(), float(ms[i]), int(la[i])) for i in range(cnt.value)]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.vfs_sim_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class SpatialHash:
+    """SpatialHash (src/compute/spatial_hash.h:9-34) on device positions."""
+
+    def __init__(self, n: int, cell_size: float):
+        h = C.c_void_p()
+        call("vfs_spatial_hash_create", int(n), float(cell_size), C.byref(h))
+        self.handle = h.value
+        self.n = int(n)
+
+    def run(self, positions_dev_ptr: int, stride_floats: int = 3, stream=None) -> None:
+        call("vfs_spatial_hash_run", self.handle, positions_dev_ptr, stride_floats, stream)
+
+    def results(self):
+        k, i, o = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        call("vfs_spatial_hash_buffers", self.handle, C.byref(k), C.byref(i), C.byref(o))
+        call("vfs_stream_synchronize", None)
+        out = []
+        for ptr in (k.value, i.value, o.value):
+            a = np.empty(self.n, np.uint32)
+            if self.n:
+                call("vfs_memcpy_d2h", a.ctypes.data, ptr, a.nbytes)
+            out.append(a)
+        return tuple(out)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.vfs_spatial_hash_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sort_pairs(keys: np.ndarray, max_value: int | None = None):
+    """GPUCountSort::Run on host data: returns (sorted_keys, permutation)."""
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    n = keys.size
+    if max_value is None:
+        max_value = int(keys.max()) if n else 0
+    ws = C.c_size_t(0)
+    call("vfs_sort_workspace_size", n, max_value, C.byref(ws))
+    dk = DeviceArray.from_host(keys)
+    di = DeviceArray(4 * n)
+    dw = DeviceArray(ws.value)
+    call("vfs_sort_pairs", di.ptr, dk.ptr, n, max_value, dw.ptr, ws.value, None)
+    call("vfs_stream_synchronize", None)
+    return dk.to_host(np.uint32, n), di.to_host(np.uint32, n)
+
+
+def scan_exclusive(values: np.ndarray) -> np.ndarray:
+    values = np.ascontiguousarray(values, dtype=np.uint32)
+    n = values.size
+    ws = C.c_size_t(0)
+    call("vfs_scan_workspace_size", n, C.byref(ws))
+    d = DeviceArray.from_host(values)
+    w = DeviceArray(ws.value)
+    call("vfs_scan_exclusive", d.ptr, n, w.ptr, ws.value, None)
+    call("vfs_stream_synchronize", None)
+    return d.to_host(np.uint32, n)
+
+
+def spatial_offsets(sorted_keys: np.ndarray, init: bool = True, prefill: np.ndarray | None = None) -> np.ndarray:
+    sorted_keys = np.ascontiguousarray(sorted_keys, dtype=np.uint32)
+    n = sorted_keys.size
+    dk = DeviceArray.from_host(sorted_keys)
+    do = DeviceArray.from_host(prefill.astype(np.uint32)) if prefill is not None else DeviceArray(4 * n)
+    call("vfs_spatial_offsets", dk.ptr, do.ptr, n, 1 if init else 0, None)
+    call("vfs_stream_synchronize", None)
+    return do.to_host(np.uint32, n)
+
+
+def reorder_f32x3(perm: np.ndarray, src: np.ndarray) -> np.ndarray:
+    perm = np.ascontiguousarray(perm, dtype=np.uint32)
+    src = np.ascontiguousarray(src, dtype=np.float32).reshape(-1)
+    n = perm.size
+    dp = DeviceArray.from_host(perm)
+    ds = DeviceArray.from_host(src)
+    dd = DeviceArray(src.nbytes)
+    call("vfs_reorder_f32x3", dp.ptr, ds.ptr, dd.ptr, n, None)
+    call("vfs_stream_synchronize", None)
+    return dd.to_host(np.float32, 3 * n).reshape(n, 3)
+
+
+class Scene:
+    """A JSON scene (LoadScene + GenericScene, src/util/scene_loader.cpp:113-140)."""
+
+    def __init__(self, path: str, resource_root: str = ""):
+        h = C.c_void_p()
+        call("vfs_scene_load", path.encode(), resource_root.encode(), C.byref(h))
+        self.handle = h.value
+        s = C.c_void_p()
+        call("vfs_scene_sim", self.handle, C.byref(s))
+        self.sim_handle = s.value
+        p = Params()
+        call("vfs_sim_get_params", self.sim_handle, C.byref(p))
+        self.params = p
+        self.n = int(p.n_particles)
+
+    def step(self, stream=None) -> None:
+        call("vfs_scene_step", self.handle, stream)
+
+    def reset(self) -> None:
+        call("vfs_scene_reset", self.handle)
+
+    def get_state(self, ids: bool = False):
+        n = self.n
+        x = np.empty(3 * n, np.float32)
+        v = np.empty(3 * n, np.float32)
+        rho = np.empty(n, np.float32)
+        pid = np.empty(n, np.uint32) if ids else None
+        call("vfs_sim_get_state", self.sim_handle, fptr(x), fptr(v), fptr(rho), uptr(pid) if ids else None)
+        return x.reshape(n, 3), v.reshape(n, 3), rho, pid
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.vfs_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,114 @@
+"""Synthetic inputs of the benchmark workloads (host-side numpy).
+
+Seeded with splitmix64 (seed 0x5EED0000 + workload index): the k-th uniform
+of a stream is (mix(seed + (k+1) * golden) >> 40) * 2^-24; Gaussians use
+Box-Muller on consecutive pairs.  Lattices are x-major with z innermost, as
+the reference seeds them (src/models/model.cpp:52-63).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+SEED_BASE = 0x5EED0000
+
+
+def splitmix64(seed: int, count: int, start: int = 0) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        k = np.arange(start + 1, start + count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, count: int, start: int = 0) -> np.ndarray:
+    """Uniform float32 in [0, 1) with 24 random bits (exactly representable)."""
+    return ((splitmix64(seed, count, start) >> np.uint64(40)).astype(np.float64) * 2.0 ** -24).astype(np.float32)
+
+
+def normal(seed: int, count: int, start: int = 0) -> np.ndarray:
+    m = (count + 1) // 2
+    u = uniform(seed, 2 * m, start).astype(np.float64)
+    u1 = np.maximum(u[0::2], 2.0 ** -25)
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+    out = np.empty(2 * m)
+    out[0::2] = z[:m]
+    out[1::2] = z[m:]
+    return out[:count]
+
+
+def lattice(counts, spacing: float, origin, jitter: float = 0.0, seed: int = 0) -> np.ndarray:
+    """(N, 3) float32 block lattice, x outer, z inner; the arithmetic matches
+    GenericScene::Reset with the spacing/jitter extension keys."""
+    nx, ny, nz = (int(c) for c in counts)
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    idx = np.stack([i.reshape(-1), j.reshape(-1), k.reshape(-1)], axis=1).astype(np.float32)
+    x = np.float32(spacing) * idx + np.asarray(origin, dtype=np.float32)[None, :]
+    if jitter:
+        u = uniform(seed, x.size).reshape(-1, 3)
+        x = x + (np.float32(2.0) * u - np.float32(1.0)) * np.float32(jitter)
+    return np.ascontiguousarray(x, dtype=np.float32)
+
+
+@dataclass
+class Workload:
+    name: str
+    n: int
+    h: float
+    dt: float
+    steps: int
+    box_size: tuple
+    box_pos: tuple
+    x: np.ndarray
+    v: np.ndarray
+    # analytic boundary shapes: (kind, params, resolution)
+    shapes: list = field(default_factory=list)
+    params_kw: dict = field(default_factory=dict)
+
+
+def dam_break(side=(32, 32, 32), spacing: float = 0.02, jitter: float = 0.001, seed: int = SEED_BASE,
+              steps: int = 100, map_res: int = 64, name: str = "wcsph_dam_break") -> Workload:
+    """Plain WCSPH dam break: lattice block with spacing 0.02 in a 1.6 x 0.8 x 0.8
+    box, h = 2 spacing, dt = 2e-4, box walls as an inverted-box volume map."""
+    h = 2.0 * spacing
+    box = (1.6, 0.8, 0.8)
+    x = lattice(side, spacing, (spacing, spacing, 0.08), jitter, seed)
+    v = np.zeros_like(x)
+    walls = (2, (0.8, 0.4, 0.4, 0.8, 0.4, 0.4), (map_res,) * 3)
+    return Workload(name, x.shape[0], h, 2e-4, steps, box, (0.0, 0.0, 0.0), x, v, [walls])
+
+
+def obstacle_1m(spacing: float = 0.006, map_res: int = 128, side=(128, 64, 128), steps: int = 500) -> Workload:
+    """1,048,576-particle lattice (128 x 64 x 128) in a 2 x 1 x 1 box with a centred
+    sphere obstacle of radius 0.2 (128^3 SDF/volume map), dt = 1e-4.  The spacing
+    is 0.006 (h = 0.012) so that the block fits the box next to the sphere."""
+    h = 2.0 * spacing
+    x = lattice(side, spacing, (spacing, spacing, spacing), 0.1 * spacing, SEED_BASE + 1)
+    v = np.zeros_like(x)
+    sphere = (1, (1.0, 0.5, 0.5, 0.2), (map_res,) * 3)
+    return Workload("wcsph_obstacle_1m", x.shape[0], h, 1e-4, steps, (2.0, 1.0, 1.0), (0.0, 0.0, 0.0), x, v,
+                    [sphere])
+
+
+def uniform_stress(n: int = 8_388_608, h: float = 0.02, map_res: int = 256, steps: int = 200) -> Workload:
+    """n particles uniform in a 4 x 2 x 2 box with N(0, 0.5) velocities, box walls."""
+    seed = SEED_BASE + 2
+    u = uniform(seed, 3 * n).reshape(n, 3)
+    x = (u * np.array([4.0, 2.0, 2.0], dtype=np.float32)).astype(np.float32)
+    v = (0.5 * normal(seed ^ 0xABCDEF, 3 * n)).astype(np.float32).reshape(n, 3)
+    walls = (2, (2.0, 1.0, 1.0, 2.0, 1.0, 1.0), (map_res,) * 3)
+    return Workload("wcsph_uniform_stress", n, h, 1e-4, steps, (4.0, 2.0, 2.0), (0.0, 0.0, 0.0), x, v, [walls])
+
+
+WORKLOADS = {
+    "dam_break_32k": lambda: dam_break(),
+    "obstacle_1m": lambda: obstacle_1m(),
+    "uniform_8m": lambda: uniform_stress(),
+    "dam_break_65k": lambda: dam_break(side=(64, 32, 32), seed=SEED_BASE + 4, steps=200,
+                                       name="wcsph_dam_break_65k"),
+}

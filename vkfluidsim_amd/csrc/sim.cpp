@@ -1,0 +1,600 @@
+// Host orchestration of the WCSPH substep (the C ABI of include/vfs_c.h).
+//
+// One substep (reference WCSPHWithBoundaryModel::Step, wcsph_with_boundary_model.cpp:81-117,
+// with SPHModel::RunSpatialHash, model.cpp:210-223) runs as
+//   sort(keys, iota) -> cell ranges -> reorder x, v (and ids)
+//   -> boundary volume + density + EOS terms
+//   -> pressure + viscosity + wall + integrate + hash of the new positions.
+// ExternalAccel is folded into the force pass (a starts at gravity), and the
+// keys of the next substep are produced by the force pass, so the standalone
+// hash kernel only runs after set_state.  The kernel sequence of a substep is
+// captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sort_internal.h"
+
+namespace vfs {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+int check_launch(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(VFS_ERR_HIP, std::string("kernel launch ") + what + ": " + hipGetErrorString(e));
+    return 0;
+}
+
+SimConsts make_consts(const vfs_params& p) {
+    SimConsts c{};
+    c.gx = p.gravity[0];
+    c.gy = p.gravity[1];
+    c.gz = p.gravity[2];
+    c.h = p.smooth_radius;
+    c.h2 = c.h * c.h;
+    c.cell = p.cell_size > 0.0f ? p.cell_size : p.smooth_radius;
+    c.rho0 = p.target_density;
+    c.B = p.stiffness;
+    c.gamma = p.expoent;
+    c.gamma_int = (p.expoent >= 0.0f && p.expoent <= 64.0f && std::floor(p.expoent) == p.expoent)
+                      ? (int32_t)p.expoent
+                      : -1;
+    c.nu10 = (2.0f * (3.0f + 2.0f)) * p.viscosity_strenght;
+    c.kwall_neg = -p.wall_stiffness;
+    c.dt = p.time_scale * p.fixed_dt / (float)p.iterations;
+    const float h = c.h;
+    const float h3 = (h * h) * h;
+    c.mass = c.rho0 * ((0.8f * h3) / 8.0f);
+    const float pi_f = 3.14159265358979323846f;
+    c.sigma = 8.0f / (pi_f * (float)std::pow((double)h, 3));
+    const float grad_scale = 48.0f / (pi_f * (float)std::pow((double)h, 4));
+    c.grad_sigma = grad_scale / h;
+    c.visc_eps_h2 = 0.01f * c.h2;
+    c.rho0_sq = c.rho0 * c.rho0;
+    for (int a = 0; a < 3; a++) {
+        c.bb_lo[a] = p.bbox_pos[a];
+        c.bb_hi[a] = p.bbox_pos[a] + p.bbox_size[a];
+    }
+    c.n = (uint32_t)p.n_particles;
+    c.fm = make_fastmod(c.n);
+    return c;
+}
+
+}  // namespace vfs
+
+using namespace vfs;
+
+struct vfs_sim {
+    vfs_params params{};
+    SimConsts consts{};
+    uint32_t n = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    float4 *x = nullptr, *v = nullptr, *xs = nullptr, *vs = nullptr, *bnd = nullptr, *accel = nullptr;
+    float* rho = nullptr;
+    float2* aux = nullptr;
+    uint32_t *pid = nullptr, *pids = nullptr, *keys = nullptr, *sorted_keys = nullptr, *perm = nullptr,
+             *cell_start = nullptr, *sort_ws = nullptr;
+    SortPlan plan{};
+    std::vector<DevObject> objs_host;
+    DevObject* objs_dev = nullptr;
+    std::vector<float*> grids;
+
+    bool keys_valid = false;
+    bool stepped = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int block = 256;
+
+    StepBuffers buffers() const {
+        StepBuffers b{};
+        b.x = x;
+        b.v = v;
+        b.pid = pid;
+        b.keys = keys;
+        b.sorted_keys = sorted_keys;
+        b.perm = perm;
+        b.cell_start = cell_start;
+        b.xs = xs;
+        b.vs = vs;
+        b.pids = pids;
+        b.rho = rho;
+        b.aux = aux;
+        b.bnd = bnd;
+        b.accel = accel;
+        b.n = n;
+        return b;
+    }
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+
+    int record_substep(hipStream_t s, StageTimer* timer) {
+        const StepBuffers b = buffers();
+        VFS_TRY(sort_keys_iota(plan, keys, sorted_keys, perm, sort_ws, s, timer));
+        if (timer) timer->begin(s, "cell_ranges");
+        VFS_TRY(launch_fill_ranges(sorted_keys, n, cell_start, nullptr, 0, s));
+        if (timer) timer->end(s);
+        if (timer) timer->begin(s, "reorder");
+        VFS_TRY(launch_reorder_state(b, s));
+        if (timer) timer->end(s);
+        if (timer) timer->begin(s, "density");
+        VFS_TRY(launch_density(b, consts, objs_dev, (uint32_t)objs_host.size(), block, s));
+        if (timer) timer->end(s);
+        if (timer) timer->begin(s, "forces_integrate");
+        VFS_TRY(launch_forces_integrate(b, consts, block, s));
+        if (timer) timer->end(s);
+        return 0;
+    }
+
+    int ensure_keys(hipStream_t s) {
+        if (keys_valid) return 0;
+        VFS_TRY(launch_hash_keys(reinterpret_cast<const float*>(x), 4, n, consts.cell, n, keys, s));
+        keys_valid = true;
+        return 0;
+    }
+
+    int ensure_graph() {
+        if (exec) return 0;
+        VFS_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const int r = record_substep(stream, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        if (r != 0) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        if (e != hipSuccess) return fail(VFS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        graph = g;
+        VFS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        return 0;
+    }
+
+    // Makes `waiter` wait for the work queued so far on `signaller`.
+    static int order_after(hipStream_t waiter, hipStream_t signaller) {
+        hipEvent_t ev;
+        VFS_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        VFS_HIP(hipEventRecord(ev, signaller));
+        VFS_HIP(hipStreamWaitEvent(waiter, ev, 0));
+        VFS_HIP(hipEventDestroy(ev));
+        return 0;
+    }
+
+    int substeps(uint32_t count, hipStream_t s) {
+        if (n == 0 || count == 0) return 0;
+        if (s != stream) VFS_TRY(order_after(s, stream));
+        VFS_TRY(ensure_keys(s));
+        if (!(params.flags & VFS_FLAG_NO_GRAPH)) {
+            VFS_TRY(ensure_graph());
+            for (uint32_t k = 0; k < count; k++) VFS_HIP(hipGraphLaunch(exec, s));
+        } else {
+            for (uint32_t k = 0; k < count; k++) VFS_TRY(record_substep(s, nullptr));
+        }
+        // later calls (state reads, set_state) synchronise the simulation stream
+        if (s != stream) VFS_TRY(order_after(stream, s));
+        stepped = true;
+        return 0;
+    }
+};
+
+namespace {
+
+template <class T>
+int dev_alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess) return fail(VFS_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
+void free_sim(vfs_sim* s) {
+    if (!s) return;
+    s->drop_graph();
+    void* ptrs[] = {s->x,    s->v,    s->xs,   s->vs,          s->bnd,  s->accel,      s->rho,
+                    s->aux,  s->pid,  s->pids, s->keys,        s->sorted_keys, s->perm, s->cell_start,
+                    s->sort_ws, s->objs_dev};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (float* g : s->grids)
+        if (g) (void)hipFree(g);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int validate_params(const vfs_params* p) {
+    VFS_CHECK(p, VFS_ERR_INVALID, "params is null");
+    VFS_CHECK(p->n_particles >= 0 && p->n_particles < (1 << 30), VFS_ERR_INVALID,
+              "n_particles must be in [0, 2^30)");
+    VFS_CHECK(p->iterations >= 1, VFS_ERR_INVALID, "iterations must be >= 1");
+    VFS_CHECK(p->smooth_radius > 0.0f && std::isfinite(p->smooth_radius), VFS_ERR_INVALID,
+              "smooth_radius must be positive");
+    VFS_CHECK(p->cell_size >= 0.0f && std::isfinite(p->cell_size), VFS_ERR_INVALID, "cell_size must be >= 0");
+    const float cell = p->cell_size > 0.0f ? p->cell_size : p->smooth_radius;
+    VFS_CHECK(cell >= p->smooth_radius, VFS_ERR_INVALID,
+              "cell_size smaller than smooth_radius would miss neighbours of the 27-cell stencil");
+    return 0;
+}
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+struct EventTimer : StageTimer {
+    struct Rec {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::string pending;
+    hipEvent_t pa = nullptr;
+    hipError_t err = hipSuccess;
+    void begin(hipStream_t s, const char* name) override {
+        pending = name;
+        if (hipEventCreate(&pa) != hipSuccess) err = hipErrorUnknown;
+        (void)hipEventRecord(pa, s);
+    }
+    void end(hipStream_t s) override {
+        hipEvent_t pb;
+        if (hipEventCreate(&pb) != hipSuccess) err = hipErrorUnknown;
+        (void)hipEventRecord(pb, s);
+        recs.push_back({pending, pa, pb});
+    }
+    ~EventTimer() override {
+        for (auto& r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* vfs_last_error(void) { return g_last_error.c_str(); }
+int vfs_abi_version(void) { return VFS_ABI_VERSION; }
+
+int vfs_device_count(int* count) {
+    if (!count) return fail(VFS_ERR_INVALID, "count is null");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    (void)hipGetLastError();
+    *count = c;
+    return 0;
+}
+
+int vfs_dev_malloc(void** ptr, size_t bytes) {
+    VFS_CHECK(ptr, VFS_ERR_INVALID, "ptr is null");
+    VFS_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    return 0;
+}
+int vfs_dev_free(void* ptr) {
+    if (ptr) VFS_HIP(hipFree(ptr));
+    return 0;
+}
+int vfs_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    if (bytes) VFS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+int vfs_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    if (bytes) VFS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+int vfs_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    if (bytes) VFS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return 0;
+}
+int vfs_memset_d32(void* dst, uint32_t value, size_t count) {
+    if (count) VFS_HIP(hipMemsetD32(reinterpret_cast<hipDeviceptr_t>(dst), (int)value, count));
+    return 0;
+}
+int vfs_stream_create(void** stream) {
+    VFS_CHECK(stream, VFS_ERR_INVALID, "stream is null");
+    hipStream_t s;
+    VFS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return 0;
+}
+int vfs_stream_destroy(void* stream) {
+    if (stream) VFS_HIP(hipStreamDestroy(as_stream(stream)));
+    return 0;
+}
+int vfs_stream_synchronize(void* stream) {
+    VFS_HIP(hipStreamSynchronize(as_stream(stream)));
+    return 0;
+}
+
+int vfs_sim_create(const vfs_params* params, const vfs_boundary_object* objects, uint32_t n_objects,
+                   vfs_sim** out) {
+    VFS_CHECK(out, VFS_ERR_INVALID, "out is null");
+    *out = nullptr;
+    VFS_TRY(validate_params(params));
+    VFS_CHECK(n_objects == 0 || objects, VFS_ERR_INVALID, "objects is null");
+    for (uint32_t k = 0; k < n_objects; k++) {
+        const auto& o = objects[k];
+        VFS_CHECK(o.sdf_grid && o.volume_map_grid, VFS_ERR_INVALID, "boundary object grid is null");
+        VFS_CHECK(o.resolution[0] >= 2 && o.resolution[1] >= 2 && o.resolution[2] >= 2, VFS_ERR_INVALID,
+                  "boundary grid resolution must be >= 2 per axis");
+        VFS_CHECK((uint64_t)o.resolution[0] * o.resolution[1] * o.resolution[2] < (1ull << 31), VFS_ERR_INVALID,
+                  "boundary grid too large");
+        VFS_CHECK(o.box_size[0] > 0 && o.box_size[1] > 0 && o.box_size[2] > 0, VFS_ERR_INVALID,
+                  "boundary box size must be positive");
+    }
+    auto* s = new vfs_sim();
+    s->params = *params;
+    s->consts = make_consts(*params);
+    s->n = (uint32_t)params->n_particles;
+    auto bail = [&](int r) {
+        free_sim(s);
+        return r;
+    };
+    if (hipGetDevice(&s->device) != hipSuccess) return bail(fail(VFS_ERR_HIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(VFS_ERR_HIP, "hipStreamCreate failed"));
+    const uint32_t n = s->n;
+    int r = 0;
+    r |= dev_alloc(&s->x, n);
+    r |= dev_alloc(&s->v, n);
+    r |= dev_alloc(&s->xs, n);
+    r |= dev_alloc(&s->vs, n);
+    r |= dev_alloc(&s->bnd, n);
+    r |= dev_alloc(&s->rho, n);
+    r |= dev_alloc(&s->aux, n);
+    r |= dev_alloc(&s->keys, n);
+    r |= dev_alloc(&s->sorted_keys, n);
+    r |= dev_alloc(&s->perm, n);
+    r |= dev_alloc(&s->cell_start, (size_t)n + 1);
+    if (params->flags & VFS_FLAG_TRACK_IDS) {
+        r |= dev_alloc(&s->pid, n);
+        r |= dev_alloc(&s->pids, n);
+    }
+    if (params->flags & VFS_FLAG_STORE_ACCEL) r |= dev_alloc(&s->accel, n);
+    s->plan = make_sort_plan(n, n ? n - 1 : 0, 8, 16);
+    r |= dev_alloc(&s->sort_ws, s->plan.workspace_words);
+    if (r) return bail(VFS_ERR_NOMEM);
+    for (uint32_t k = 0; k < n_objects; k++) {
+        const auto& o = objects[k];
+        DevObject d{};
+        std::memcpy(d.T, o.transform, sizeof(d.T));
+        std::memcpy(d.Rm, o.rotation, sizeof(d.Rm));
+        for (int a = 0; a < 3; a++) {
+            d.box_pos[a] = o.box_pos[a];
+            d.box_size[a] = o.box_size[a];
+            d.res[a] = o.resolution[a];
+        }
+        const size_t cnt = (size_t)o.resolution[0] * o.resolution[1] * o.resolution[2];
+        float *gs = nullptr, *gv = nullptr;
+        if (dev_alloc(&gs, cnt) || dev_alloc(&gv, cnt)) {
+            if (gs) (void)hipFree(gs);
+            return bail(VFS_ERR_NOMEM);
+        }
+        s->grids.push_back(gs);
+        s->grids.push_back(gv);
+        if (hipMemcpyAsync(gs, o.sdf_grid, cnt * 4, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipMemcpyAsync(gv, o.volume_map_grid, cnt * 4, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+            hipStreamSynchronize(s->stream) != hipSuccess)
+            return bail(fail(VFS_ERR_HIP, "grid upload failed"));
+        d.sdf = gs;
+        d.vm = gv;
+        s->objs_host.push_back(d);
+    }
+    if (n_objects) {
+        if (dev_alloc(&s->objs_dev, n_objects)) return bail(VFS_ERR_NOMEM);
+        if (hipMemcpyAsync(s->objs_dev, s->objs_host.data(), n_objects * sizeof(DevObject), hipMemcpyHostToDevice,
+                           s->stream) != hipSuccess ||
+            hipStreamSynchronize(s->stream) != hipSuccess)
+            return bail(fail(VFS_ERR_HIP, "object upload failed"));
+    }
+    // zero state, iota ids
+    if (n) {
+        if (hipMemsetAsync(s->x, 0, (size_t)n * 16, s->stream) != hipSuccess ||
+            hipMemsetAsync(s->v, 0, (size_t)n * 16, s->stream) != hipSuccess ||
+            hipMemsetAsync(s->rho, 0, (size_t)n * 4, s->stream) != hipSuccess ||
+            hipMemsetAsync(s->bnd, 0, (size_t)n * 16, s->stream) != hipSuccess)
+            return bail(fail(VFS_ERR_HIP, "state clear failed"));
+        if (s->accel && hipMemsetAsync(s->accel, 0, (size_t)n * 16, s->stream) != hipSuccess)
+            return bail(fail(VFS_ERR_HIP, "state clear failed"));
+        if (s->pid && launch_iota(s->pid, n, s->stream)) return bail(VFS_ERR_HIP);
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(VFS_ERR_HIP, "sync failed"));
+    }
+    *out = s;
+    return 0;
+}
+
+int vfs_sim_destroy(vfs_sim* sim) {
+    if (sim) {
+        (void)hipStreamSynchronize(sim->stream);
+        free_sim(sim);
+    }
+    return 0;
+}
+
+int vfs_sim_set_state(vfs_sim* sim, const float* positions, const float* velocities, uint32_t offset,
+                      int64_t count) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    const uint32_t n = sim->n;
+    if (count < 0) count = (int64_t)n - (int64_t)offset;
+    VFS_CHECK(count >= 0 && (uint64_t)offset + (uint64_t)count <= n, VFS_ERR_INVALID, "range out of bounds");
+    if (count == 0) return 0;
+    VFS_CHECK(positions && velocities, VFS_ERR_INVALID, "positions/velocities null");
+    std::vector<float4> hx((size_t)count), hv((size_t)count);
+    for (int64_t i = 0; i < count; i++) {
+        hx[i] = make_float4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 0.0f);
+        hv[i] = make_float4(velocities[3 * i], velocities[3 * i + 1], velocities[3 * i + 2], 0.0f);
+    }
+    VFS_HIP(hipMemcpyAsync(sim->x + offset, hx.data(), (size_t)count * 16, hipMemcpyHostToDevice, sim->stream));
+    VFS_HIP(hipMemcpyAsync(sim->v + offset, hv.data(), (size_t)count * 16, hipMemcpyHostToDevice, sim->stream));
+    VFS_HIP(hipMemsetAsync(sim->rho + offset, 0, (size_t)count * 4, sim->stream));
+    if (sim->pid) VFS_TRY(launch_iota(sim->pid, n, sim->stream));
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    sim->keys_valid = false;
+    return 0;
+}
+
+int vfs_sim_set_params(vfs_sim* sim, const vfs_params* params) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    VFS_TRY(validate_params(params));
+    VFS_CHECK(params->n_particles == sim->params.n_particles, VFS_ERR_INVALID, "n_particles cannot change");
+    VFS_CHECK(params->flags == sim->params.flags, VFS_ERR_INVALID, "flags cannot change");
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    const float old_cell = sim->consts.cell;
+    sim->params = *params;
+    sim->consts = make_consts(*params);
+    if (sim->consts.cell != old_cell) sim->keys_valid = false;
+    sim->drop_graph();
+    return 0;
+}
+
+int vfs_sim_get_params(const vfs_sim* sim, vfs_params* params) {
+    VFS_CHECK(sim && params, VFS_ERR_INVALID, "null argument");
+    *params = sim->params;
+    return 0;
+}
+
+int vfs_sim_substeps(vfs_sim* sim, uint32_t n, void* stream) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    hipStream_t s = stream ? as_stream(stream) : sim->stream;
+    return sim->substeps(n, s);
+}
+
+int vfs_sim_synchronize(vfs_sim* sim) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    return 0;
+}
+
+int vfs_sim_step(vfs_sim* sim, void* stream) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    return vfs_sim_substeps(sim, (uint32_t)sim->params.iterations, stream);
+}
+
+int vfs_sim_get_state(vfs_sim* sim, float* positions, float* velocities, float* densities,
+                      uint32_t* particle_ids) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    VFS_CHECK(!particle_ids || sim->pid, VFS_ERR_STATE, "particle ids need VFS_FLAG_TRACK_IDS");
+    const uint32_t n = sim->n;
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    std::vector<float4> tmp(n);
+    if (positions && n) {
+        VFS_HIP(hipMemcpy(tmp.data(), sim->x, (size_t)n * 16, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; i++) {
+            positions[3 * (size_t)i] = tmp[i].x;
+            positions[3 * (size_t)i + 1] = tmp[i].y;
+            positions[3 * (size_t)i + 2] = tmp[i].z;
+        }
+    }
+    if (velocities && n) {
+        VFS_HIP(hipMemcpy(tmp.data(), sim->v, (size_t)n * 16, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; i++) {
+            velocities[3 * (size_t)i] = tmp[i].x;
+            velocities[3 * (size_t)i + 1] = tmp[i].y;
+            velocities[3 * (size_t)i + 2] = tmp[i].z;
+        }
+    }
+    if (densities && n) VFS_HIP(hipMemcpy(densities, sim->rho, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (particle_ids && n) VFS_HIP(hipMemcpy(particle_ids, sim->pid, (size_t)n * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int vfs_sim_get_hash(vfs_sim* sim, uint32_t* sorted_keys, uint32_t* sorted_indices, uint32_t* offsets) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    VFS_CHECK(sim->stepped, VFS_ERR_STATE, "no substep has run yet");
+    const uint32_t n = sim->n;
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    if (sorted_keys && n) VFS_HIP(hipMemcpy(sorted_keys, sim->sorted_keys, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (sorted_indices && n) VFS_HIP(hipMemcpy(sorted_indices, sim->perm, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (offsets && n) {
+        uint32_t* tmp = nullptr;
+        VFS_TRY(dev_alloc(&tmp, n));
+        int r = launch_offsets_from_ranges(sim->cell_start, n, tmp, sim->stream);
+        if (r == 0 && hipStreamSynchronize(sim->stream) != hipSuccess) r = fail(VFS_ERR_HIP, "sync failed");
+        if (r == 0 && hipMemcpy(offsets, tmp, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            r = fail(VFS_ERR_HIP, "copy failed");
+        (void)hipFree(tmp);
+        return r;
+    }
+    return 0;
+}
+
+int vfs_sim_get_boundary(vfs_sim* sim, float* boundary_density, float* boundary_gradient) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    const uint32_t n = sim->n;
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    std::vector<float4> tmp(n);
+    if (n) VFS_HIP(hipMemcpy(tmp.data(), sim->bnd, (size_t)n * 16, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; i++) {
+        if (boundary_density) boundary_density[i] = tmp[i].w;
+        if (boundary_gradient) {
+            boundary_gradient[3 * (size_t)i] = tmp[i].x;
+            boundary_gradient[3 * (size_t)i + 1] = tmp[i].y;
+            boundary_gradient[3 * (size_t)i + 2] = tmp[i].z;
+        }
+    }
+    return 0;
+}
+
+int vfs_sim_device_buffers(vfs_sim* sim, vfs_device_buffers* out) {
+    VFS_CHECK(sim && out, VFS_ERR_INVALID, "null argument");
+    out->positions = reinterpret_cast<float*>(sim->x);
+    out->velocities = reinterpret_cast<float*>(sim->v);
+    out->densities = sim->rho;
+    out->accelerations = reinterpret_cast<float*>(sim->accel);
+    out->particle_ids = sim->pid;
+    out->sorted_keys = sim->sorted_keys;
+    out->sorted_indices = sim->perm;
+    out->cell_start = sim->cell_start;
+    out->n_particles = sim->n;
+    return 0;
+}
+
+int vfs_sim_profile_stages(vfs_sim* sim, uint32_t n, float* ms, uint32_t* launches, const char** names,
+                           uint32_t max_stages, uint32_t* n_stages) {
+    VFS_CHECK(sim && n_stages, VFS_ERR_INVALID, "null argument");
+    static const char* kNames[] = {"sort_prepare", "sort_histogram", "sort_pass", "cell_ranges",
+                                   "reorder",      "density",        "forces_integrate"};
+    constexpr uint32_t kN = sizeof(kNames) / sizeof(kNames[0]);
+    *n_stages = 0;
+    if (sim->n == 0 || n == 0) return 0;
+    VFS_TRY(sim->ensure_keys(sim->stream));
+    EventTimer timer;
+    for (uint32_t k = 0; k < n; k++) VFS_TRY(sim->record_substep(sim->stream, &timer));
+    sim->stepped = true;
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    if (timer.err != hipSuccess) return fail(VFS_ERR_HIP, "event creation failed");
+    double tot[kN] = {0};
+    uint32_t cnt[kN] = {0};
+    for (auto& r : timer.recs) {
+        float t = 0.0f;
+        VFS_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        for (uint32_t i = 0; i < kN; i++)
+            if (r.name == kNames[i]) {
+                tot[i] += t;
+                cnt[i]++;
+            }
+    }
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < kN && m < max_stages; i++) {
+        if (ms) ms[m] = (float)(tot[i] / n);
+        if (launches) launches[m] = cnt[i] / n;
+        if (names) names[m] = kNames[i];
+        m++;
+    }
+    *n_stages = m;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,549 @@
+// Spatial-hash kernels for gfx950: cell-key hashing, a stable onesweep LSD
+// radix sort (wave64 ballot ranking staged in LDS, decoupled look-back across
+// tiles), cell ranges / reference-format offsets, reorder, and a standalone
+// decoupled look-back exclusive scan.
+//
+// Replaces the reference compute layer (luihabl/VkFluidSim):
+//   UpdateSpatialHash        assets/shaders/simulation/spatial_hash/update_spatial_hash.slang:18-28
+//   ClearCounts..CopyBack    assets/shaders/simulation/spatial_hash/sort.slang:13-55
+//   ScanBlock/ScanCombine    assets/shaders/simulation/spatial_hash/scan.slang:13-83
+//   InitOffsets/Calculate..  assets/shaders/simulation/spatial_hash/spatial_offsets.slang:10-31
+//   Reorder                  assets/shaders/simulation/spatial_hash/reorder.slang:11-19
+// The reference counting sort scatters with atomics, so the order inside a
+// bucket is not defined there; this sort is stable (ascending input index
+// inside a key), which is the canonical order the oracle uses.
+#include "sort_internal.h"
+
+namespace vfs {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagInclusive = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1u;
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const uint32_t lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ __forceinline__ uint32_t ld_acquire_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release_agent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t& total) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[w] = x;
+    __syncthreads();
+    uint32_t wave_off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; i++) {
+        const uint32_t t = s_tmp[i];
+        if (i < w) wave_off += t;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return wave_off + x - v;
+}
+
+// ------------------------------------------------------------------ kernels
+
+// UpdateSpatialHash: key = HashCell3D(floor(x / cell)) % table_size.
+__global__ __launch_bounds__(256) void k_hash_keys(const float* __restrict__ pos, uint32_t stride,
+                                                   uint32_t n, float cell, FastMod fm,
+                                                   uint32_t* __restrict__ keys) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* p = pos + (size_t)i * stride;
+    const int32_t cx = floor_to_cell(p[0] / cell);
+    const int32_t cy = floor_to_cell(p[1] / cell);
+    const int32_t cz = floor_to_cell(p[2] / cell);
+    keys[i] = fastmod(hash_cell(cx, cy, cz), fm);
+}
+
+__global__ __launch_bounds__(256) void k_zero_words(uint32_t* __restrict__ a, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        a[i] = 0u;
+}
+
+// Digit histograms of every pass in one read of the keys.
+template <int BITS>
+__global__ __launch_bounds__(256) void k_radix_histogram(const uint32_t* __restrict__ keys, uint32_t n,
+                                                         int passes, uint32_t* __restrict__ hist) {
+    constexpr int RADIX = 1 << BITS;
+    __shared__ uint32_t sh[kMaxPasses * RADIX];
+    for (int i = threadIdx.x; i < passes * RADIX; i += kThreads) sh[i] = 0;
+    __syncthreads();
+    for (uint32_t idx = blockIdx.x * kThreads + threadIdx.x; idx < n; idx += gridDim.x * kThreads) {
+        const uint32_t k = keys[idx];
+        for (int p = 0; p < passes; p++) atomicAdd(&sh[p * RADIX + ((k >> (p * BITS)) & (RADIX - 1))], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * RADIX; i += kThreads) {
+        const uint32_t c = sh[i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+// One onesweep pass over digit `pass`.  Each block claims the next tile in
+// launch order, ranks its keys with wave ballots (stable: wave-major, then
+// item-major, then lane order = input order), publishes its per-digit counts
+// and resolves its global offsets by decoupled look-back over earlier tiles.
+template <int BITS, int ITEMS>
+__global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__ keys_in,
+                                                    const uint32_t* __restrict__ vals_in,
+                                                    uint32_t* __restrict__ keys_out,
+                                                    uint32_t* __restrict__ vals_out, uint32_t n, int shift,
+                                                    const uint32_t* __restrict__ hist,
+                                                    uint32_t* __restrict__ status,
+                                                    uint32_t* __restrict__ tile_counter) {
+    constexpr int RADIX = 1 << BITS;
+    constexpr int DPT = RADIX / kThreads;  // digits per thread
+    constexpr int TILE = kThreads * ITEMS;
+    static_assert(DPT >= 1, "radix must be >= 256");
+
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave[kWaves][RADIX];
+    __shared__ uint32_t s_local[RADIX];
+    __shared__ int32_t s_global[RADIX];
+    __shared__ uint32_t s_keys[TILE];
+    __shared__ uint32_t s_vals[TILE];
+    __shared__ uint32_t s_tmp[kWaves];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    for (int i = tid; i < kWaves * RADIX; i += kThreads) (&s_wave[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * (uint32_t)TILE;
+    const uint32_t wave_base = base + (uint32_t)w * 64u * ITEMS;
+
+    uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint32_t idx = wave_base + (uint32_t)it * 64u + (uint32_t)lane;
+        const bool valid = idx < n;
+        key[it] = valid ? keys_in[idx] : 0xFFFFFFFFu;
+        val[it] = vals_in ? (valid ? vals_in[idx] : 0u) : idx;
+    }
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint32_t idx = wave_base + (uint32_t)it * 64u + (uint32_t)lane;
+        const bool valid = idx < n;
+        const uint32_t d = (key[it] >> shift) & (RADIX - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < BITS; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        if (valid) {
+            const uint32_t before = (uint32_t)__popcll(peers & lt);
+            const uint32_t prior = s_wave[w][d];
+            rank[it] = prior + before;
+            if (before == 0) s_wave[w][d] = prior + (uint32_t)__popcll(peers);
+        }
+    }
+    __syncthreads();
+
+    // Per digit: exclusive offsets across waves and the tile total.
+    uint32_t tcount[DPT];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int dd = 0; dd < DPT; dd++) {
+        const int d = tid * DPT + dd;
+        uint32_t s = 0;
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ww++) {
+            const uint32_t c = s_wave[ww][d];
+            s_wave[ww][d] = s;
+            s += c;
+        }
+        tcount[dd] = s;
+        tsum += s;
+        // Publish the aggregate before looking back so that later tiles progress.
+        st_release_agent(&status[(size_t)tile * RADIX + d], (tile == 0 ? kFlagInclusive : kFlagAggregate) | s);
+    }
+    // Local (in-tile) digit starts.
+    uint32_t tot;
+    uint32_t lrun = block_exclusive_scan(tsum, s_tmp, tot);
+    // Global digit starts from the pass histogram.
+    uint32_t hv[DPT];
+    uint32_t hsum = 0;
+#pragma unroll
+    for (int dd = 0; dd < DPT; dd++) {
+        hv[dd] = hist[tid * DPT + dd];
+        hsum += hv[dd];
+    }
+    uint32_t grun = block_exclusive_scan(hsum, s_tmp, tot);
+
+#pragma unroll
+    for (int dd = 0; dd < DPT; dd++) {
+        const int d = tid * DPT + dd;
+        uint32_t excl = 0;
+        if (tile > 0) {
+            int64_t t = (int64_t)tile - 1;
+            while (true) {
+                const uint32_t s = ld_acquire_agent(&status[(size_t)t * RADIX + d]);
+                const uint32_t flag = s & ~kValueMask;
+                if (flag == 0) continue;
+                excl += s & kValueMask;
+                if (flag == kFlagInclusive) break;
+                t--;
+            }
+            st_release_agent(&status[(size_t)tile * RADIX + d], kFlagInclusive | (excl + tcount[dd]));
+        }
+        s_local[d] = lrun;
+        s_global[d] = (int32_t)(grun + excl) - (int32_t)lrun;
+        lrun += tcount[dd];
+        grun += hv[dd];
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int it = 0; it < ITEMS; it++) {
+        const uint32_t idx = wave_base + (uint32_t)it * 64u + (uint32_t)lane;
+        if (idx < n) {
+            const uint32_t d = (key[it] >> shift) & (RADIX - 1);
+            const uint32_t lpos = s_local[d] + s_wave[w][d] + rank[it];
+            s_keys[lpos] = key[it];
+            s_vals[lpos] = val[it];
+        }
+    }
+    __syncthreads();
+
+    const uint32_t tile_valid = (n - base) < (uint32_t)TILE ? (n - base) : (uint32_t)TILE;
+    for (uint32_t i = tid; i < tile_valid; i += kThreads) {
+        const uint32_t k = s_keys[i];
+        const uint32_t d = (k >> shift) & (RADIX - 1);
+        const uint32_t dst = (uint32_t)(s_global[d] + (int32_t)i);
+        keys_out[dst] = k;
+        vals_out[dst] = s_vals[i];
+    }
+}
+
+// Cell ranges from sorted keys: cell_start[k] = first i with sorted[i] >= k,
+// for k in [0, n] (so bucket k is [cell_start[k], cell_start[k+1])), and/or
+// the reference offsets table (first index of key k, n if empty; init=0 only
+// writes the starts of present keys).  Thread i owns keys (sorted[i-1], sorted[i]];
+// a wave fills long runs cooperatively.
+__global__ __launch_bounds__(256) void k_fill_ranges(const uint32_t* __restrict__ sk, uint32_t n,
+                                                     uint32_t* __restrict__ cell_start,
+                                                     uint32_t* __restrict__ offsets, int init) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    uint32_t lo = 1, hi = 0;  // empty
+    if (i <= n) {
+        lo = (i == 0) ? 0u : sk[i - 1] + 1u;
+        hi = (i == n) ? n : sk[i];
+        if (hi > n) hi = n;
+    }
+    bool has = lo <= hi;
+    if (offsets && !init) {
+        if (has && i < n && hi < n) offsets[hi] = i;
+        if (!cell_start) return;
+    }
+    while (true) {
+        const uint64_t m = __ballot(has);
+        if (m == 0) break;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        const uint32_t L = __shfl(lo, leader, 64);
+        const uint32_t H = __shfl(hi, leader, 64);
+        const uint32_t V = __shfl(i, leader, 64);
+        for (uint32_t k = L + (uint32_t)lane; k <= H; k += 64u) {
+            if (cell_start) cell_start[k] = V;
+            if (offsets && init && k < n) offsets[k] = (k == H && V < n) ? V : n;
+        }
+        if (lane == leader) has = false;
+    }
+}
+
+// Reference offsets from cell ranges (empty bucket -> n).
+__global__ __launch_bounds__(256) void k_offsets_from_ranges(const uint32_t* __restrict__ cs, uint32_t n,
+                                                             uint32_t* __restrict__ offsets) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t a = cs[k], b = cs[k + 1];
+    offsets[k] = a < b ? a : n;
+}
+
+__global__ __launch_bounds__(256) void k_reorder_f32x3(const uint32_t* __restrict__ perm,
+                                                       const float* __restrict__ src,
+                                                       float* __restrict__ dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = perm[i];
+    dst[3 * (size_t)i + 0] = src[3 * (size_t)s + 0];
+    dst[3 * (size_t)i + 1] = src[3 * (size_t)s + 1];
+    dst[3 * (size_t)i + 2] = src[3 * (size_t)s + 2];
+}
+
+__global__ __launch_bounds__(256) void k_iota(uint32_t* __restrict__ a, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_copy_words(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                    uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// Decoupled look-back exclusive scan (in place).  64-bit status words:
+// high half = flag (1 aggregate, 2 inclusive), low half = value mod 2^32.
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_scan_lookback(uint32_t* __restrict__ data, uint32_t n,
+                                                       unsigned long long* __restrict__ status,
+                                                       uint32_t* __restrict__ tile_counter) {
+    constexpr int TILE = kThreads * ITEMS;
+    __shared__ uint32_t s_data[TILE];
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_prefix;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * (uint32_t)TILE;
+    for (int i = tid; i < TILE; i += kThreads) {
+        const uint32_t idx = base + i;
+        s_data[i] = idx < n ? data[idx] : 0u;
+    }
+    __syncthreads();
+    uint32_t local[ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        local[j] = s_data[tid * ITEMS + j];
+        sum += local[j];
+    }
+    uint32_t total;
+    const uint32_t excl_thread = block_exclusive_scan(sum, s_tmp, total);
+    if (tid == 0) {
+        const unsigned long long agg = ((unsigned long long)(tile == 0 ? 2u : 1u) << 32) | total;
+        __hip_atomic_store(&status[tile], agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t prefix = 0;
+        if (tile > 0) {
+            int64_t t = (int64_t)tile - 1;
+            while (true) {
+                const unsigned long long s =
+                    __hip_atomic_load(&status[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t flag = (uint32_t)(s >> 32);
+                if (flag == 0) continue;
+                prefix += (uint32_t)s;
+                if (flag == 2) break;
+                t--;
+            }
+            __hip_atomic_store(&status[tile], (2ull << 32) | (uint32_t)(prefix + total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_prefix = prefix;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + excl_thread;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        s_data[tid * ITEMS + j] = run;
+        run += local[j];
+    }
+    __syncthreads();
+    for (int i = tid; i < TILE; i += kThreads) {
+        const uint32_t idx = base + i;
+        if (idx < n) data[idx] = s_data[i];
+    }
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t block = 256) {
+    return (uint32_t)((n + block - 1) / block);
+}
+
+int bit_width(uint32_t v) {
+    int b = 0;
+    while (v) {
+        b++;
+        v >>= 1;
+    }
+    return b;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ host launchers
+
+SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int items) {
+    SortPlan p{};
+    p.n = n;
+    p.bits = bits_per_pass;
+    p.items = items;
+    const int kb = bit_width(max_value);
+    p.passes = kb == 0 ? 1 : (kb + bits_per_pass - 1) / bits_per_pass;
+    const uint32_t tile = (uint32_t)kThreads * (uint32_t)items;
+    p.num_tiles = n == 0 ? 0 : (n + tile - 1) / tile;
+    const size_t radix = (size_t)1 << bits_per_pass;
+    // layout: tmp keys/vals x2, hist, counters, status
+    size_t off = 0;
+    auto take = [&](size_t words) {
+        const size_t o = off;
+        off += (words + 63) & ~(size_t)63;
+        return o;
+    };
+    p.off_tmp[0] = take(n);
+    p.off_tmp[1] = take(n);
+    p.off_tmp[2] = take(n);
+    p.off_tmp[3] = take(n);
+    p.off_hist = take(kMaxPasses * radix);
+    p.off_counters = take(kMaxPasses);
+    p.off_status = take((size_t)p.passes * p.num_tiles * radix);
+    p.zero_begin = p.off_hist;
+    p.workspace_words = off;
+    return p;
+}
+
+int valid_sort_config(int bits, int items) {
+    return (bits == 8 || bits == 9 || bits == 10) && (items == 8 || items == 12 || items == 16);
+}
+
+template <int BITS, int ITEMS>
+static void launch_pass(const uint32_t* ki, const uint32_t* vi, uint32_t* ko, uint32_t* vo, uint32_t n, int shift,
+                        const uint32_t* hist, uint32_t* status, uint32_t* counter, uint32_t tiles,
+                        hipStream_t s) {
+    hipLaunchKernelGGL((k_radix_pass<BITS, ITEMS>), dim3(tiles), dim3(kThreads), 0, s, ki, vi, ko, vo, n, shift,
+                       hist, status, counter);
+}
+
+template <int BITS>
+static void launch_pass_items(int items, const uint32_t* ki, const uint32_t* vi, uint32_t* ko, uint32_t* vo,
+                              uint32_t n, int shift, const uint32_t* hist, uint32_t* status, uint32_t* counter,
+                              uint32_t tiles, hipStream_t s) {
+    if (items == 8)
+        launch_pass<BITS, 8>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, s);
+    else if (items == 12)
+        launch_pass<BITS, 12>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, s);
+    else
+        launch_pass<BITS, 16>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, s);
+}
+
+int sort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
+                   uint32_t* workspace, hipStream_t s, StageTimer* timer) {
+    const uint32_t n = p.n;
+    if (n == 0) return 0;
+    const size_t radix = (size_t)1 << p.bits;
+    uint32_t* hist = workspace + p.off_hist;
+    uint32_t* counters = workspace + p.off_counters;
+    uint32_t* status = workspace + p.off_status;
+    const size_t zero_words = p.workspace_words - p.zero_begin;
+    if (timer) timer->begin(s, "sort_prepare");
+    hipLaunchKernelGGL(k_zero_words, dim3(std::min<uint32_t>(grid_for(zero_words), 2048u)), dim3(256), 0, s,
+                       workspace + p.zero_begin, zero_words);
+    VFS_TRY(check_launch("sort_prepare"));
+    if (timer) timer->end(s);
+    if (timer) timer->begin(s, "sort_histogram");
+    const uint32_t hblocks = std::max<uint32_t>(1u, std::min<uint32_t>(grid_for(n, 256u * 16u), 1024u));
+    if (p.bits == 8)
+        hipLaunchKernelGGL(k_radix_histogram<8>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist);
+    else if (p.bits == 9)
+        hipLaunchKernelGGL(k_radix_histogram<9>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist);
+    else
+        hipLaunchKernelGGL(k_radix_histogram<10>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist);
+    VFS_TRY(check_launch("sort_histogram"));
+    if (timer) timer->end(s);
+    uint32_t* tmpk[2] = {workspace + p.off_tmp[0], workspace + p.off_tmp[2]};
+    uint32_t* tmpv[2] = {workspace + p.off_tmp[1], workspace + p.off_tmp[3]};
+    for (int pass = 0; pass < p.passes; pass++) {
+        const uint32_t* ki = pass == 0 ? keys_in : tmpk[(pass - 1) & 1];
+        const uint32_t* vi = pass == 0 ? nullptr : tmpv[(pass - 1) & 1];
+        const bool last = pass == p.passes - 1;
+        uint32_t* ko = last ? keys_out : tmpk[pass & 1];
+        uint32_t* vo = last ? perm_out : tmpv[pass & 1];
+        uint32_t* st = status + (size_t)pass * p.num_tiles * radix;
+        if (timer) timer->begin(s, "sort_pass");
+        if (p.bits == 8)
+            launch_pass_items<8>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
+                                 counters + pass, p.num_tiles, s);
+        else if (p.bits == 9)
+            launch_pass_items<9>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
+                                 counters + pass, p.num_tiles, s);
+        else
+            launch_pass_items<10>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
+                                  counters + pass, p.num_tiles, s);
+        VFS_TRY(check_launch("sort_pass"));
+        if (timer) timer->end(s);
+    }
+    return 0;
+}
+
+int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,
+                     uint32_t* keys, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_hash_keys, dim3(grid_for(n)), dim3(256), 0, s, pos, stride, n, cell,
+                       make_fastmod(table_size), keys);
+    return check_launch("hash_keys");
+}
+
+int launch_fill_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t* cell_start, uint32_t* offsets, int init,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_ranges, dim3(grid_for((uint64_t)n + 1)), dim3(256), 0, s, sorted_keys, n, cell_start,
+                       offsets, init);
+    return check_launch("fill_ranges");
+}
+
+int launch_offsets_from_ranges(const uint32_t* cell_start, uint32_t n, uint32_t* offsets, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_offsets_from_ranges, dim3(grid_for(n)), dim3(256), 0, s, cell_start, n, offsets);
+    return check_launch("offsets_from_ranges");
+}
+
+int launch_reorder_f32x3(const uint32_t* perm, const float* src, float* dst, uint32_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_reorder_f32x3, dim3(grid_for(n)), dim3(256), 0, s, perm, src, dst, n);
+    return check_launch("reorder_f32x3");
+}
+
+int launch_iota(uint32_t* a, uint32_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(256), 0, s, a, n);
+    return check_launch("iota");
+}
+
+int launch_copy_words(const uint32_t* src, uint32_t* dst, uint32_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_copy_words, dim3(grid_for(n)), dim3(256), 0, s, src, dst, n);
+    return check_launch("copy_words");
+}
+
+size_t scan_workspace_words(uint32_t n) {
+    const uint32_t tile = kThreads * 8;
+    const size_t tiles = (n + tile - 1) / tile;
+    return 64 + 2 * tiles;
+}
+
+int launch_scan_exclusive(uint32_t* data, uint32_t n, uint32_t* workspace, hipStream_t s) {
+    if (n == 0) return 0;
+    const uint32_t tile = kThreads * 8;
+    const uint32_t tiles = (n + tile - 1) / tile;
+    const size_t words = scan_workspace_words(n);
+    hipLaunchKernelGGL(k_zero_words, dim3(std::min<uint32_t>(grid_for(words), 1024u)), dim3(256), 0, s, workspace,
+                       words);
+    VFS_TRY(check_launch("scan_prepare"));
+    uint32_t* counter = workspace;
+    auto* status = reinterpret_cast<unsigned long long*>(workspace + 64);
+    hipLaunchKernelGGL(k_scan_lookback<8>, dim3(tiles), dim3(256), 0, s, data, n, status, counter);
+    return check_launch("scan_lookback");
+}
+
+}  // namespace vfs

@@ -1,0 +1,86 @@
+// Internal host launchers of the spatial-hash kernels (sort.hip) and the WCSPH
+// kernels (wcsph.hip).  Not part of the C ABI.
+#pragma once
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+
+#include "vfs_common.h"
+
+namespace vfs {
+
+constexpr int kMaxPasses = 4;
+
+// Optional per-stage event timing (used by vfs_sim_profile_stages).
+struct StageTimer {
+    virtual void begin(hipStream_t s, const char* name) = 0;
+    virtual void end(hipStream_t s) = 0;
+    virtual ~StageTimer() = default;
+};
+
+// Word offsets into one u32 workspace.
+struct SortPlan {
+    uint32_t n;
+    int bits;        // digit width (8, 9 or 10)
+    int items;       // keys per thread (tile = 256 * items)
+    int passes;
+    uint32_t num_tiles;
+    size_t off_tmp[4];
+    size_t off_hist, off_counters, off_status;
+    size_t zero_begin;
+    size_t workspace_words;
+};
+
+SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int items);
+int valid_sort_config(int bits, int items);
+
+// Stable sort of (keys_in, iota): keys_out = sorted keys, perm_out[i] = input
+// index of the i-th smallest (key, index).  keys_in is not modified.
+int sort_keys_iota(const SortPlan& plan, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
+                   uint32_t* workspace, hipStream_t s, StageTimer* timer);
+
+int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,
+                     uint32_t* keys, hipStream_t s);
+int launch_fill_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t* cell_start, uint32_t* offsets,
+                       int init, hipStream_t s);
+int launch_offsets_from_ranges(const uint32_t* cell_start, uint32_t n, uint32_t* offsets, hipStream_t s);
+int launch_reorder_f32x3(const uint32_t* perm, const float* src, float* dst, uint32_t n, hipStream_t s);
+int launch_iota(uint32_t* a, uint32_t n, hipStream_t s);
+int launch_copy_words(const uint32_t* src, uint32_t* dst, uint32_t n, hipStream_t s);
+size_t scan_workspace_words(uint32_t n);
+int launch_scan_exclusive(uint32_t* data, uint32_t n, uint32_t* workspace, hipStream_t s);
+
+// ---- WCSPH substep kernels (wcsph.hip)
+struct StepBuffers {
+    // state in the current order (input of a substep, output of the previous one)
+    float4* x;
+    float4* v;
+    uint32_t* pid;       // nullable
+    uint32_t* keys;      // hash keys of x
+    // sorted / reordered
+    uint32_t* sorted_keys;
+    uint32_t* perm;
+    uint32_t* cell_start;  // n + 1
+    float4* xs;
+    float4* vs;
+    uint32_t* pids;      // nullable
+    // per-particle results
+    float* rho;
+    float2* aux;         // (p / rho^2, mass / max(rho, 1e-6))
+    float4* bnd;         // (grad psi xyz, psi)
+    float4* accel;       // nullable
+    uint32_t n;
+};
+
+int launch_reorder_state(const StepBuffers& b, hipStream_t s);
+int launch_density(const StepBuffers& b, const SimConsts& c, const DevObject* objs, uint32_t n_obj, int block,
+                   hipStream_t s);
+int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, int block, hipStream_t s);
+
+// Volume-map/SDF grid generator for analytic shapes (FP64 on the device).
+int launch_volume_map(int kind, const double* shape_params, const uint32_t* res, double support_radius,
+                      double tolerance, const float* dmin, const double* step, const double* gl_x,
+                      const double* gl_w, float* sdf_out, float* vm_out, hipStream_t s);
+
+}  // namespace vfs

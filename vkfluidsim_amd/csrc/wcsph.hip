@@ -1,0 +1,495 @@
+// WCSPH substep kernels for gfx950: reorder of the particle state, boundary
+// volume + density + equation of state, and the fused pressure + viscosity +
+// wall + symplectic-Euler update that also hashes the new positions for the
+// next substep.  Plus the analytic-shape volume-map generator.
+//
+// Arithmetic follows the reference shaders (luihabl/VkFluidSim,
+// assets/shaders/simulation/wcsph_with_boundary_model.slang and
+// kernels/kernels_3d.slang) operation by operation; the library is compiled
+// with -ffp-contract=off so that every float operation rounds once, as in the
+// CPU oracle.  Neighbour enumeration visits the 27 hashed cells in the
+// reference order (z outer, y, x inner; spatial_hash_3d.slang:3-10) and each
+// bucket in ascending sorted index, so sums are accumulated in the reference
+// order.  When the three x-adjacent cells of a row hash to consecutive keys
+// their buckets are contiguous in sorted order and are walked as one range.
+#include "sort_internal.h"
+
+namespace vfs {
+
+namespace {
+
+// CubicSpline, kernels_3d.slang:62-76
+__device__ __forceinline__ float kernel_w(float r, const SimConsts& c) {
+    const float q = r / c.h;
+    if (q <= 1.0f) {
+        if (q > 0.5f) {
+            const float t = 1.0f - q;
+            return (c.sigma * 2.0f) * ((t * t) * t);
+        }
+        const float q2 = q * q;
+        const float q3 = q2 * q;
+        return c.sigma * ((6.0f * q3 - 6.0f * q2) + 1.0f);
+    }
+    return 0.0f;
+}
+
+// GradCubicSpline, kernels_3d.slang:78-91 (grad_sigma carries the extra 1/h)
+__device__ __forceinline__ float kernel_dw(float r, const SimConsts& c) {
+    const float q = r / c.h;
+    if ((r > 1e-9f) && (q <= 1.0f)) {
+        if (q > 0.5f) {
+            const float t = 1.0f - q;
+            return (-c.grad_sigma) * (t * t);
+        }
+        return (c.grad_sigma * q) * (3.0f * q - 2.0f);
+    }
+    return 0.0f;
+}
+
+// (rho / rho0)^gamma: binary powering in double for an integral exponent,
+// rounded once to float; general exponents use the double pow.
+__device__ __forceinline__ float eos_pow(float x, const SimConsts& c) {
+    if (c.gamma_int >= 0) {
+        unsigned e = (unsigned)c.gamma_int;
+        double base = (double)x, result = 1.0;
+        while (e) {
+            if (e & 1u) result *= base;
+            base *= base;
+            e >>= 1;
+        }
+        return (float)result;
+    }
+    return (float)pow((double)x, (double)c.gamma);
+}
+
+// PressureFromDensityEOS, slang:199-203
+__device__ __forceinline__ float eos(float rho, const SimConsts& c) {
+    rho = fmaxf(rho, c.rho0);
+    return c.B * (eos_pow(rho / c.rho0, c) - 1.0f);
+}
+
+__device__ __forceinline__ uint32_t cell_key(int32_t cx, int32_t cy, int32_t cz, int dx, int dy, int dz,
+                                             const SimConsts& c) {
+    return fastmod(hash_cell((int32_t)((uint32_t)cx + (uint32_t)dx), (int32_t)((uint32_t)cy + (uint32_t)dy),
+                             (int32_t)((uint32_t)cz + (uint32_t)dz)),
+                   c.fm);
+}
+
+// SampleGrid, slang:51-66, base cell clamped to [0, n-2].
+__device__ float sample_grid(const float* __restrict__ grid, const uint32_t* res, const float* uvw) {
+    uint32_t ci[3];
+    float chi[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float x = ((float)res[a] - 1.0f) * uvw[a];
+        const float f = floorf(x);
+        const uint32_t hi = res[a] >= 2u ? res[a] - 2u : 0u;
+        uint32_t cc;
+        if (f >= 0.0f)
+            cc = f <= (float)hi ? (uint32_t)f : hi;
+        else
+            cc = 0u;
+        ci[a] = cc;
+        chi[a] = 2.0f * (x - (float)cc) - 1.0f;
+    }
+    const uint32_t sx = 1u, sy = res[0], sz = res[0] * res[1];
+    const uint32_t b0 = ci[0] + res[0] * ci[1] + sz * ci[2];
+    const float mx = 1.0f + (-1.0f) * chi[0], px = 1.0f + chi[0];
+    const float my = 1.0f + (-1.0f) * chi[1], py = 1.0f + chi[1];
+    const float mz = 1.0f + (-1.0f) * chi[2], pz = 1.0f + chi[2];
+    // offsets (0,0,0),(0,1,0),(0,0,1),(0,1,1),(1,0,0),(1,1,0),(1,0,1),(1,1,1)
+    float val = 0.0f;
+    val += grid[b0] * ((mx * my) * mz);
+    val += grid[b0 + sy] * ((mx * py) * mz);
+    val += grid[b0 + sz] * ((mx * my) * pz);
+    val += grid[b0 + sy + sz] * ((mx * py) * pz);
+    val += grid[b0 + sx] * ((px * my) * mz);
+    val += grid[b0 + sx + sy] * ((px * py) * mz);
+    val += grid[b0 + sx + sz] * ((px * my) * pz);
+    val += grid[b0 + sx + sy + sz] * ((px * py) * pz);
+    return val / 8.0f;
+}
+
+__device__ __forceinline__ void to_uvw(const DevObject& o, const float* p, float* uvw) {
+    uvw[0] = (p[0] - o.box_pos[0]) / o.box_size[0];
+    uvw[1] = (p[1] - o.box_pos[1]) / o.box_size[1];
+    uvw[2] = (p[2] - o.box_pos[2]) / o.box_size[2];
+}
+
+__device__ __forceinline__ void mat_vec(const float* M, const float* v, float* out) {
+#pragma unroll
+    for (int r = 0; r < 3; r++) out[r] = ((M[r] * v[0] + M[4 + r] * v[1]) + M[8 + r] * v[2]) + M[12 + r] * 1.0f;
+}
+
+// CalculateBoundaryVolume for one particle, slang:102-151.
+__device__ void boundary_volume(const DevObject* __restrict__ objs, uint32_t n_obj, const float* pos,
+                                const SimConsts& c, float& psi, float* grad) {
+    psi = 0.0f;
+    grad[0] = grad[1] = grad[2] = 0.0f;
+    for (uint32_t oi = 0; oi < n_obj; oi++) {
+        const DevObject& o = objs[oi];
+        float lp[3];
+        mat_vec(o.T, pos, lp);
+        bool outside = false;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            outside |= (lp[a] < o.box_pos[a]) || (lp[a] > o.box_pos[a] + o.box_size[a]);
+        if (outside) continue;
+        float uvw[3];
+        to_uvw(o, lp, uvw);
+        const float dist = sample_grid(o.sdf, o.res, uvw);
+        if (!(dist >= 0.0f && dist < c.h)) continue;
+        const float volume = sample_grid(o.vm, o.res, uvw);
+        if (!(volume > 0.0f)) continue;
+        // GradientVector, slang:68-85
+        const float eps = 0.1f * c.h;
+        float nrm[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float xp[3] = {lp[0], lp[1], lp[2]}, xm[3] = {lp[0], lp[1], lp[2]};
+            xp[a] = lp[a] + eps;
+            xm[a] = lp[a] - eps;
+            float u[3];
+            to_uvw(o, xp, u);
+            const float v1 = sample_grid(o.sdf, o.res, u);
+            to_uvw(o, xm, u);
+            const float v0 = sample_grid(o.sdf, o.res, u);
+            nrm[a] = (v1 - v0) / (2.0f * eps);
+        }
+        const float modn = sqrtf((nrm[0] * nrm[0] + nrm[1] * nrm[1]) + nrm[2] * nrm[2]);
+        if (!(modn > 1e-9f)) continue;
+        float rn[3];
+        mat_vec(o.Rm, nrm, rn);
+#pragma unroll
+        for (int a = 0; a < 3; a++) rn[a] = rn[a] / modn;
+        const float d = fmaxf(dist + 0.125f * c.h, 0.5f * c.h);
+        float xij[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float xb = pos[a] - d * rn[a];
+            xij[a] = pos[a] - xb;
+        }
+        const float xm = sqrtf((xij[0] * xij[0] + xij[1] * xij[1]) + xij[2] * xij[2]);
+        psi += volume * kernel_w(xm, c);
+        const float vg = volume * kernel_dw(xm, c);
+#pragma unroll
+        for (int a = 0; a < 3; a++) grad[a] += vg * (xm > 0.0f ? xij[a] / xm : 0.0f);
+    }
+}
+
+// ------------------------------------------------------------------ kernels
+
+__global__ __launch_bounds__(256) void k_reorder_state(StepBuffers b) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.n) return;
+    const uint32_t s = b.perm[i];
+    b.xs[i] = b.x[s];
+    b.vs[i] = b.v[s];
+    if (b.pid) b.pids[i] = b.pid[s];
+}
+
+// Boundary volume, CalculateDensities (slang:153-197) and the per-particle
+// EOS terms the force pass reads for every neighbour.
+__global__ __launch_bounds__(256) void k_density(StepBuffers b, SimConsts c, const DevObject* __restrict__ objs,
+                                                 uint32_t n_obj) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 p4 = b.xs[i];
+    const float pos[3] = {p4.x, p4.y, p4.z};
+    float psi, g[3];
+    boundary_volume(objs, n_obj, pos, c, psi, g);
+
+    const int32_t cx = floor_to_cell(pos[0] / c.cell);
+    const int32_t cy = floor_to_cell(pos[1] / c.cell);
+    const int32_t cz = floor_to_cell(pos[2] / c.cell);
+    const float4* __restrict__ xs = b.xs;
+    const uint32_t* __restrict__ cs = b.cell_start;
+    float density = 0.0f;
+    for (int dz = -1; dz <= 1; dz++) {
+        for (int dy = -1; dy <= 1; dy++) {
+            uint32_t k[3];
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) k[dx] = cell_key(cx, cy, cz, dx - 1, dy, dz, c);
+            uint32_t ranges[6];
+            int nr;
+            if (k[1] == k[0] + 1u && k[2] == k[1] + 1u) {
+                ranges[0] = cs[k[0]];
+                ranges[1] = cs[k[2] + 1u];
+                nr = 1;
+            } else {
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    ranges[2 * t] = cs[k[t]];
+                    ranges[2 * t + 1] = cs[k[t] + 1u];
+                }
+                nr = 3;
+            }
+            for (int rr = 0; rr < nr; rr++) {
+                const uint32_t jb = ranges[2 * rr], je = ranges[2 * rr + 1];
+                for (uint32_t j = jb; j < je; j++) {
+                    const float4 q = xs[j];
+                    const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
+                    const float r2 = (ox * ox + oy * oy) + oz * oz;
+                    if (r2 > c.h2) continue;
+                    const float r = sqrtf(r2);
+                    density += c.mass * kernel_w(r, c);
+                }
+            }
+        }
+    }
+    density += c.rho0 * psi;
+    b.rho[i] = density;
+    const float p = eos(density, c);
+    const float pd = density > 0.0f ? p / (density * density) : 0.0f;
+    const float mrho = c.mass / fmaxf(density, 1e-6f);
+    b.aux[i] = make_float2(pd, mrho);
+    b.bnd[i] = make_float4(g[0], g[1], g[2], psi);
+}
+
+// CalculatePressureAccel (slang:205-271) + CalculateViscousAccel (slang:273-334)
+// in one neighbour walk with separate accumulators, then WallAcceleration and
+// UpdatePositions (slang:336-360), then UpdateSpatialHash of the new position.
+__global__ __launch_bounds__(256) void k_forces_integrate(StepBuffers b, SimConsts c) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 p4 = b.xs[i];
+    const float4 v4 = b.vs[i];
+    const float di = b.rho[i];
+    const float pdi = b.aux[i].x;
+    const float pi = eos(di, c);
+
+    const int32_t cx = floor_to_cell(p4.x / c.cell);
+    const int32_t cy = floor_to_cell(p4.y / c.cell);
+    const int32_t cz = floor_to_cell(p4.z / c.cell);
+    const float4* __restrict__ xs = b.xs;
+    const float4* __restrict__ vs = b.vs;
+    const float2* __restrict__ aux = b.aux;
+    const uint32_t* __restrict__ cs = b.cell_start;
+    float pfx = 0.0f, pfy = 0.0f, pfz = 0.0f;
+    float vfx = 0.0f, vfy = 0.0f, vfz = 0.0f;
+    for (int dz = -1; dz <= 1; dz++) {
+        for (int dy = -1; dy <= 1; dy++) {
+            uint32_t k[3];
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) k[dx] = cell_key(cx, cy, cz, dx - 1, dy, dz, c);
+            uint32_t ranges[6];
+            int nr;
+            if (k[1] == k[0] + 1u && k[2] == k[1] + 1u) {
+                ranges[0] = cs[k[0]];
+                ranges[1] = cs[k[2] + 1u];
+                nr = 1;
+            } else {
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    ranges[2 * t] = cs[k[t]];
+                    ranges[2 * t + 1] = cs[k[t] + 1u];
+                }
+                nr = 3;
+            }
+            for (int rr = 0; rr < nr; rr++) {
+                const uint32_t jb = ranges[2 * rr], je = ranges[2 * rr + 1];
+                for (uint32_t j = jb; j < je; j++) {
+                    if (j == i) continue;
+                    const float4 q = xs[j];
+                    const float xx = p4.x - q.x, xy = p4.y - q.y, xz = p4.z - q.z;
+                    const float r2 = (xx * xx + xy * xy) + xz * xz;
+                    if (r2 > c.h2) continue;
+                    const float r = sqrtf(r2);
+                    float nx, ny, nz;
+                    if (r > 0.0f) {
+                        nx = xx / r;
+                        ny = xy / r;
+                        nz = xz / r;
+                    } else {
+                        nx = 0.0f;
+                        ny = 1.0f;
+                        nz = 0.0f;
+                    }
+                    const float G = kernel_dw(r, c);
+                    const float2 aj = aux[j];
+                    const float s = (c.mass * (pdi + aj.x)) * G;
+                    pfx += s * nx;
+                    pfy += s * ny;
+                    pfz += s * nz;
+                    const float4 vj = vs[j];
+                    const float ux = v4.x - vj.x, uy = v4.y - vj.y, uz = v4.z - vj.z;
+                    const float sv = aj.y * ((ux * xx + uy * xy) + uz * xz);
+                    const float den = r2 + c.visc_eps_h2;
+                    vfx += (sv * (nx * G)) / den;
+                    vfy += (sv * (ny * G)) / den;
+                    vfz += (sv * (nz * G)) / den;
+                }
+            }
+        }
+    }
+    const float pdb = di > 0.0f ? pi / c.rho0_sq : 0.0f;
+    const float sb = (pdi + pdb) * c.rho0;
+    const float4 bg = b.bnd[i];
+    pfx += sb * bg.x;
+    pfy += sb * bg.y;
+    pfz += sb * bg.z;
+
+    float ax = c.gx + (-pfx), ay = c.gy + (-pfy), az = c.gz + (-pfz);
+    ax += c.nu10 * vfx;
+    ay += c.nu10 * vfy;
+    az += c.nu10 * vfz;
+    // WallAcceleration
+    const float dfx = fminf(p4.x - c.bb_lo[0], 0.0f) - fminf(c.bb_hi[0] - p4.x, 0.0f);
+    const float dfy = fminf(p4.y - c.bb_lo[1], 0.0f) - fminf(c.bb_hi[1] - p4.y, 0.0f);
+    const float dfz = fminf(p4.z - c.bb_lo[2], 0.0f) - fminf(c.bb_hi[2] - p4.z, 0.0f);
+    const bool wall = fabsf(dfx) > 1e-9f || fabsf(dfy) > 1e-9f || fabsf(dfz) > 1e-9f;
+    ax += wall ? c.kwall_neg * dfx : 0.0f;
+    ay += wall ? c.kwall_neg * dfy : 0.0f;
+    az += wall ? c.kwall_neg * dfz : 0.0f;
+    // symplectic Euler
+    const float nvx = v4.x + ax * c.dt, nvy = v4.y + ay * c.dt, nvz = v4.z + az * c.dt;
+    const float nxx = p4.x + nvx * c.dt, nxy = p4.y + nvy * c.dt, nxz = p4.z + nvz * c.dt;
+    b.x[i] = make_float4(nxx, nxy, nxz, 0.0f);
+    b.v[i] = make_float4(nvx, nvy, nvz, 0.0f);
+    if (b.accel) b.accel[i] = make_float4(ax, ay, az, 0.0f);
+    if (b.pid) b.pid[i] = b.pids[i];
+    b.keys[i] = fastmod(hash_cell(floor_to_cell(nxx / c.cell), floor_to_cell(nxy / c.cell),
+                                  floor_to_cell(nxz / c.cell)),
+                        c.fm);
+}
+
+// ---------------------------------------------------------- volume maps
+struct ShapeArgs {
+    int kind;
+    double p[6];
+};
+struct GLTable {
+    double x[16];
+    double w[16];
+};
+
+__device__ double shape_sdf(const ShapeArgs& s, float qx, float qy, float qz) {
+    const double px = qx, py = qy, pz = qz;
+    if (s.kind == VFS_SHAPE_SPHERE) {
+        const double dx = px - s.p[0], dy = py - s.p[1], dz = pz - s.p[2];
+        return sqrt(dx * dx + dy * dy + dz * dz) - s.p[3];
+    }
+    const double ax = fabs(px - s.p[0]) - s.p[3], ay = fabs(py - s.p[1]) - s.p[4], az = fabs(pz - s.p[2]) - s.p[5];
+    const double ox = fmax(ax, 0.0), oy = fmax(ay, 0.0), oz = fmax(az, 0.0);
+    const double outside = sqrt(ox * ox + oy * oy + oz * oz);
+    const double inside = fmin(fmax(ax, fmax(ay, az)), 0.0);
+    const double d = outside + inside;
+    return s.kind == VFS_SHAPE_BOX_WALLS ? -d : d;
+}
+
+// Cubic spline of src/util/kernel.cpp:10-34 (FP64).
+__device__ __forceinline__ double kernel_w_f64(double r, double radius, double factor) {
+    const double q = r / radius;
+    if (q <= 1.0) {
+        if (q <= 0.5) {
+            const double q2 = q * q, q3 = q2 * q;
+            return factor * (6.0 * q3 - 6.0 * q2 + 1.0);
+        }
+        return factor * (2.0 * pow(1.0 - q, 3.0));
+    }
+    return 0.0;
+}
+
+// One grid node per thread: SDF value (mesh_sdf.cpp:28-45) and the volume map
+// integral of volume_map.cpp:9-54 with the 16^3 Gauss-Legendre rule of
+// gaussian_quadrature.cpp:2293-2333.
+__global__ __launch_bounds__(128) void k_volume_map(ShapeArgs sh, uint32_t rx, uint32_t ry, uint32_t rz, float sr,
+                                                    double tolerance, float d0, float d1, float d2, double s0,
+                                                    double s1, double s2, GLTable gl, float* __restrict__ sdf_out,
+                                                    float* __restrict__ vm_out) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t total = (uint64_t)rx * ry * rz;
+    if (idx >= total) return;
+    const uint32_t i = (uint32_t)(idx % rx);
+    const uint32_t j = (uint32_t)((idx / rx) % ry);
+    const uint32_t k = (uint32_t)(idx / ((uint64_t)rx * ry));
+    const float x0 = (float)(s0 * (double)i + (double)d0);
+    const float x1 = (float)(s1 * (double)j + (double)d1);
+    const float x2 = (float)(s2 * (double)k + (double)d2);
+    sdf_out[idx] = (float)(shape_sdf(sh, x0, x1, x2) - tolerance);
+
+    const double dist = shape_sdf(sh, x0, x1, x2);
+    if (dist > 2.0 * (double)sr) {
+        vm_out[idx] = 0.0f;
+        return;
+    }
+    const double radius = (double)sr;
+    const double factor = 8.0 / (M_PI * radius * radius * radius);
+    const double w_zero = kernel_w_f64(0.0, radius, factor);
+    const double c0 = 0.5 * (double)(sr - (-sr));
+    const double c1 = 0.5 * (double)(sr + (-sr));
+    const float sr2 = sr * sr;
+    double integral = 0.0;
+    for (int a = 0; a < 16; a++) {
+        const double wi = gl.w[a];
+        const float xa = (float)(c0 * gl.x[a] + c1);
+        for (int bb = 0; bb < 16; bb++) {
+            const double wij = wi * gl.w[bb];
+            const float xb = (float)(c0 * gl.x[bb] + c1);
+            for (int cc = 0; cc < 16; cc++) {
+                const double wijk = wij * gl.w[cc];
+                const float xc = (float)(c0 * gl.x[cc] + c1);
+                double val;
+                if ((xa * xa + xb * xb) + xc * xc > sr2) {
+                    val = 0.0;
+                } else {
+                    const double di = shape_sdf(sh, x0 + xa, x1 + xb, x2 + xc);
+                    if (di <= 0.0)
+                        val = 1.0;
+                    else if (di < radius)
+                        val = kernel_w_f64(di, radius, factor) / w_zero;
+                    else
+                        val = 0.0;
+                }
+                integral += wijk * val;
+            }
+        }
+    }
+    integral *= c0 * c0 * c0;
+    vm_out[idx] = (float)(0.8 * integral);
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t block) { return (uint32_t)((n + block - 1) / block); }
+
+}  // namespace
+
+int launch_reorder_state(const StepBuffers& b, hipStream_t s) {
+    if (b.n == 0) return 0;
+    hipLaunchKernelGGL(k_reorder_state, dim3(grid_for(b.n, 256)), dim3(256), 0, s, b);
+    return check_launch("reorder_state");
+}
+
+int launch_density(const StepBuffers& b, const SimConsts& c, const DevObject* objs, uint32_t n_obj, int block,
+                   hipStream_t s) {
+    if (b.n == 0) return 0;
+    (void)block;
+    hipLaunchKernelGGL(k_density, dim3(grid_for(b.n, 256)), dim3(256), 0, s, b, c, objs, n_obj);
+    return check_launch("density");
+}
+
+int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, int block, hipStream_t s) {
+    if (b.n == 0) return 0;
+    (void)block;
+    hipLaunchKernelGGL(k_forces_integrate, dim3(grid_for(b.n, 256)), dim3(256), 0, s, b, c);
+    return check_launch("forces_integrate");
+}
+
+int launch_volume_map(int kind, const double* shape_params, const uint32_t* res, double support_radius,
+                      double tolerance, const float* dmin, const double* step, const double* gl_x,
+                      const double* gl_w, float* sdf_out, float* vm_out, hipStream_t s) {
+    ShapeArgs sh;
+    sh.kind = kind;
+    for (int i = 0; i < 6; i++) sh.p[i] = shape_params[i];
+    GLTable gl;
+    for (int i = 0; i < 16; i++) {
+        gl.x[i] = gl_x[i];
+        gl.w[i] = gl_w[i];
+    }
+    const uint64_t total = (uint64_t)res[0] * res[1] * res[2];
+    hipLaunchKernelGGL(k_volume_map, dim3(grid_for(total, 128)), dim3(128), 0, s, sh, res[0], res[1], res[2],
+                       (float)support_radius, tolerance, dmin[0], dmin[1], dmin[2], step[0], step[1], step[2], gl,
+                       sdf_out, vm_out);
+    return check_launch("volume_map");
+}
+
+}  // namespace vfs
