@@ -104,9 +104,15 @@ def main() -> int:
     ms_per_step = 1e3 * elapsed / args.steps
     value = w.n * args.steps / elapsed
 
-    stages = sim.profile(args.profile_steps)
     x, v, rho, _ = sim.get_state()
     finite = bool(np.isfinite(x).all() and np.isfinite(v).all() and np.isfinite(rho).all())
+    lo = np.asarray(w.box_pos, np.float32)
+    hi = lo + np.asarray(w.box_size, np.float32)
+    end_state = {"substeps_run": args.warmup + args.steps, "finite": finite,
+                 "max_speed": float(np.linalg.norm(v, axis=1).max()) if w.n else 0.0,
+                 "outside_box": int(np.any((x < lo) | (x > hi), axis=1).sum()),
+                 "rho_mean": float(rho.mean()) if w.n else 0.0}
+    stages = sim.profile(args.profile_steps)
     sim.close()
 
     stage_ms = {name: ms for name, ms, _ in stages}
@@ -145,7 +151,7 @@ def main() -> int:
         "stages_ms": {k: round(v, 5) for k, v in stage_ms.items()},
         "stage_launches": launches,
         "event_sum_ms_per_step": step_ms_events,
-        "state_finite": finite,
+        "end_state": end_state,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, grids, args.cpu_steps)

@@ -97,6 +97,15 @@ typedef struct vfs_device_buffers {
     uint32_t n_particles;
 } vfs_device_buffers;
 
+/* Launch configuration of a simulation.  Every setting produces bit-identical
+ * results; only the speed changes. */
+typedef struct vfs_tuning {
+    int32_t sort_bits;   /* radix digit width of the key sort: 8, 9 or 10 */
+    int32_t sort_items;  /* keys per thread of a sort tile: 8, 12 or 16 */
+    int32_t block_size;  /* threads per block of the neighbour kernels: 128 or 256 */
+    int32_t stage_rows;  /* 1: stage stencil rows of positions in LDS; 0: read them through L1/L2 */
+} vfs_tuning;
+
 typedef struct vfs_sim vfs_sim;
 typedef struct vfs_spatial_hash vfs_spatial_hash;
 
@@ -154,6 +163,13 @@ int vfs_sim_get_hash(vfs_sim *sim, uint32_t *sorted_keys, uint32_t *sorted_indic
  * (CalculateBoundaryVolume outputs, wcsph_with_boundary_model.slang:102-151). */
 int vfs_sim_get_boundary(vfs_sim *sim, float *boundary_density, float *boundary_gradient);
 int vfs_sim_device_buffers(vfs_sim *sim, vfs_device_buffers *out);
+int vfs_sim_get_tuning(const vfs_sim *sim, vfs_tuning *tuning);
+int vfs_sim_set_tuning(vfs_sim *sim, const vfs_tuning *tuning);
+/* Autotuner: times every vfs_tuning on `substeps_per_trial` substeps from the
+ * current state (restored before each trial and at the end, so the scene does
+ * not advance), keeps the fastest and returns it with its mean ms per substep.  No counterpart in the reference (its launch shape
+ * is fixed: 256 threads, atomic counting sort). */
+int vfs_sim_autotune(vfs_sim *sim, uint32_t substeps_per_trial, vfs_tuning *best, float *best_ms_per_substep);
 /* Runs n substeps eagerly (advancing the state) with a HIP event pair around
  * every kernel on the simulation stream.  For each stage writes the mean time
  * per substep in ms (summed over its launches), the launches per substep and

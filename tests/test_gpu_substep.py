@@ -229,7 +229,47 @@ def test_profile_stages_report_every_kernel():
     sim.set_state(w.x, w.v)
     stages = sim.profile(3)
     names = [s[0] for s in stages]
-    assert names == ["sort_prepare", "sort_histogram", "sort_pass", "cell_ranges", "reorder", "density",
+    assert names == ["sort_prepare", "sort_histogram", "sort_pass", "reorder", "cell_ranges", "density",
                      "forces_integrate"]
     assert all(ms > 0 for _, ms, _ in stages)
     assert dict((s[0], s[2]) for s in stages)["sort_pass"] == 2  # 12-bit keys at n = 4096
+
+
+TUNINGS = [(b, it, bs, st) for b in (8, 9, 10) for it in (8, 12, 16) for bs in (128, 256) for st in (0, 1)]
+
+
+def test_every_tuning_is_bit_identical():
+    """Sort radix/tile, block size and LDS row staging change speed, never bits."""
+    w = CF.dam_break()
+    grids = _grids(w)
+    orc = H.OracleSim(_params(w), grids)
+    orc.set_state(w.x, w.v)
+    orc.substeps(6)
+    rx, rv, rrho, rpid, _ = orc.get_state()
+    rk, rp, ro = orc.get_hash()
+    for t in TUNINGS:
+        sim = V.Sim(_params(w), grids)
+        sim.set_tuning(*t)
+        assert sim.get_tuning().as_tuple() == t
+        sim.set_state(w.x, w.v)
+        sim.substeps(6)
+        x, v, rho, pid = sim.get_state(ids=True)
+        k, p, o = sim.get_hash()
+        assert np.array_equal(x.astype(np.float64), rx), t
+        assert np.array_equal(v.astype(np.float64), rv), t
+        assert np.array_equal(rho.astype(np.float64), rrho), t
+        assert np.array_equal(pid, rpid) and np.array_equal(k, rk) and np.array_equal(p, rp), t
+        assert np.array_equal(o, ro), t
+        sim.close()
+
+
+def test_autotune_keeps_results_and_returns_a_valid_tuning():
+    w = CF.dam_break(side=(16, 16, 16), map_res=32)
+    grids = _grids(w)
+    a = V.Sim(_params(w), grids)
+    a.set_state(w.x, w.v)
+    best, ms = a.autotune(2)
+    assert ms > 0 and best.as_tuple() in TUNINGS
+    assert a.get_tuning().as_tuple() == best.as_tuple()
+    with pytest.raises(V.VfsError):
+        a.set_tuning(7, 16, 256, 0)

@@ -12,10 +12,10 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (BoundaryObject, DeviceArray, Params, VfsError, VFS_FLAG_NO_GRAPH, VFS_FLAG_STORE_ACCEL,
+from ._lib import (BoundaryObject, DeviceArray, Params, Tuning, VfsError, VFS_FLAG_NO_GRAPH, VFS_FLAG_STORE_ACCEL,
                    VFS_FLAG_TRACK_IDS, VFS_SHAPE_BOX, VFS_SHAPE_BOX_WALLS, VFS_SHAPE_SPHERE, call, fptr, lib, uptr)
 
-__all__ = ["Sim", "Params", "BoundaryGrid", "SpatialHash", "VfsError", "make_params", "volume_map",
+__all__ = ["Sim", "Params", "Tuning", "BoundaryGrid", "SpatialHash", "VfsError", "make_params", "volume_map",
            "sort_pairs", "scan_exclusive", "spatial_offsets", "reorder_f32x3", "Scene", "lib",
            "VFS_FLAG_TRACK_IDS", "VFS_FLAG_NO_GRAPH", "VFS_FLAG_STORE_ACCEL",
            "VFS_SHAPE_BOX", "VFS_SHAPE_SPHERE", "VFS_SHAPE_BOX_WALLS", "device_count"]
@@ -148,6 +148,22 @@ class Sim:
         g = np.empty(3 * n, np.float32)
         call("vfs_sim_get_boundary", self.handle, fptr(psi), fptr(g))
         return psi, g.reshape(n, 3)
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        call("vfs_sim_get_tuning", self.handle, C.byref(t))
+        return t
+
+    def set_tuning(self, sort_bits=8, sort_items=16, block_size=256, stage_rows=0) -> None:
+        t = Tuning(sort_bits, sort_items, block_size, stage_rows)
+        call("vfs_sim_set_tuning", self.handle, C.byref(t))
+
+    def autotune(self, substeps_per_trial: int = 5):
+        """Fastest launch configuration on the current scene: (Tuning, ms per substep)."""
+        t = Tuning()
+        ms = C.c_float(0)
+        call("vfs_sim_autotune", self.handle, int(substeps_per_trial), C.byref(t), C.byref(ms))
+        return t, ms.value
 
     def device_buffers(self) -> _lib.DeviceBuffers:
         b = _lib.DeviceBuffers()

@@ -67,6 +67,15 @@ class BoundaryObject(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """vfs_tuning: launch configuration (bit-identical results for every setting)."""
+    _fields_ = [("sort_bits", C.c_int32), ("sort_items", C.c_int32), ("block_size", C.c_int32),
+                ("stage_rows", C.c_int32)]
+
+    def as_tuple(self):
+        return (self.sort_bits, self.sort_items, self.block_size, self.stage_rows)
+
+
 class DeviceBuffers(C.Structure):
     _fields_ = [
         ("positions", C.c_void_p),
@@ -117,6 +126,9 @@ SIGNATURES = {
     "vfs_sim_get_hash": (_i32, [_vp, _up, _up, _up]),
     "vfs_sim_get_boundary": (_i32, [_vp, _fp, _fp]),
     "vfs_sim_device_buffers": (_i32, [_vp, C.POINTER(DeviceBuffers)]),
+    "vfs_sim_get_tuning": (_i32, [_vp, C.POINTER(Tuning)]),
+    "vfs_sim_set_tuning": (_i32, [_vp, C.POINTER(Tuning)]),
+    "vfs_sim_autotune": (_i32, [_vp, _u32, C.POINTER(Tuning), C.POINTER(C.c_float)]),
     "vfs_sim_profile_stages": (_i32, [_vp, _u32, _fp, _up, C.POINTER(C.c_char_p), _u32, _up]),
     "vfs_spatial_hash_create": (_i32, [_u32, C.c_float, C.POINTER(_vp)]),
     "vfs_spatial_hash_set_cell_size": (_i32, [_vp, C.c_float]),

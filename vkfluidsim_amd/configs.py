@@ -83,16 +83,28 @@ def dam_break(side=(32, 32, 32), spacing: float = 0.02, jitter: float = 0.001, s
     return Workload(name, x.shape[0], h, 2e-4, steps, box, (0.0, 0.0, 0.0), x, v, [walls])
 
 
-def obstacle_1m(spacing: float = 0.006, map_res: int = 128, side=(128, 64, 128), steps: int = 500) -> Workload:
+def obstacle_1m(spacing: float = 0.006, map_res: int = 128, side=(128, 64, 128), steps: int = 500,
+                walls: bool = False) -> Workload:
     """1,048,576-particle lattice (128 x 64 x 128) in a 2 x 1 x 1 box with a centred
-    sphere obstacle of radius 0.2 (128^3 SDF/volume map), dt = 1e-4.  The spacing
-    is 0.006 (h = 0.012) so that the block fits the box next to the sphere."""
+    sphere obstacle of radius 0.2 (128^3 SDF/volume map), dt = 1e-4, box walls as
+    the penalty force.  The spacing is 0.006 (h = 0.012) so that the block fits
+    the box next to the sphere.
+
+    The lattice starts at 0.8 rho0, so it falls freely (zero pressure) for about
+    320 substeps and then hits the penalty floor; with the model's 1/h-scaled
+    kernel gradient the impact blows the scene up.  A resolved box-wall volume
+    map (walls=True, node spacing <= h/2) does not help: particles seeded h/2
+    from two walls get psi ~ 1 and are kicked out at once.  Timings are taken in
+    the first ~250 substeps."""
     h = 2.0 * spacing
     x = lattice(side, spacing, (spacing, spacing, spacing), 0.1 * spacing, SEED_BASE + 1)
     v = np.zeros_like(x)
-    sphere = (1, (1.0, 0.5, 0.5, 0.2), (map_res,) * 3)
+    shapes = [(1, (1.0, 0.5, 0.5, 0.2), (map_res,) * 3)]
+    if walls:
+        res = tuple(int(np.ceil((e + 16.0 * h) / (0.5 * h))) + 1 for e in (2.0, 1.0, 1.0))
+        shapes.append((2, (1.0, 0.5, 0.5, 1.0, 0.5, 0.5), res))
     return Workload("wcsph_obstacle_1m", x.shape[0], h, 1e-4, steps, (2.0, 1.0, 1.0), (0.0, 0.0, 0.0), x, v,
-                    [sphere])
+                    shapes)
 
 
 def uniform_stress(n: int = 8_388_608, h: float = 0.02, map_res: int = 256, steps: int = 200) -> Workload:
@@ -108,6 +120,7 @@ def uniform_stress(n: int = 8_388_608, h: float = 0.02, map_res: int = 256, step
 WORKLOADS = {
     "dam_break_32k": lambda: dam_break(),
     "obstacle_1m": lambda: obstacle_1m(),
+    "obstacle_1m_wall_map": lambda: obstacle_1m(walls=True),
     "uniform_8m": lambda: uniform_stress(),
     "dam_break_65k": lambda: dam_break(side=(64, 32, 32), seed=SEED_BASE + 4, steps=200,
                                        name="wcsph_dam_break_65k"),

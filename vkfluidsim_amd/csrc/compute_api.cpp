@@ -92,8 +92,9 @@ int vfs_spatial_hash_run(vfs_spatial_hash* h, const float* positions_dev, uint32
     VFS_CHECK(positions_dev, VFS_ERR_INVALID, "positions is null");
     hipStream_t s = as_stream(stream);
     VFS_TRY(launch_hash_keys(positions_dev, stride_floats, h->n, h->cell_size, h->n, h->raw_keys, s));
-    VFS_TRY(sort_keys_iota(h->plan, h->raw_keys, h->keys, h->indices, h->ws, s, nullptr));
-    VFS_TRY(launch_fill_ranges(h->keys, h->n, nullptr, h->offsets, 1, s));
+    // InitOffsets is folded into the sort's preparation kernel (offsets[k] = n)
+    VFS_TRY(sort_keys_iota(h->plan, h->raw_keys, h->keys, h->indices, h->ws, s, nullptr, h->offsets, h->n, h->n));
+    VFS_TRY(launch_key_starts(h->keys, h->n, h->offsets, s));
     return 0;
 }
 
@@ -158,7 +159,8 @@ int vfs_spatial_offsets(const uint32_t* sorted_keys_dev, uint32_t* offsets_dev, 
                         void* stream) {
     if (n == 0) return 0;
     VFS_CHECK(sorted_keys_dev && offsets_dev, VFS_ERR_INVALID, "null argument");
-    return launch_fill_ranges(sorted_keys_dev, n, nullptr, offsets_dev, init ? 1 : 0, as_stream(stream));
+    if (init) VFS_TRY(launch_fill_value(offsets_dev, n, n, as_stream(stream)));
+    return launch_key_starts(sorted_keys_dev, n, offsets_dev, as_stream(stream));
 }
 
 int vfs_reorder_f32x3(const uint32_t* sorted_indices_dev, const float* src_dev, float* dst_dev, uint32_t n,

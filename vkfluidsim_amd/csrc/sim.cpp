@@ -85,7 +85,7 @@ struct vfs_sim {
     float* rho = nullptr;
     float2* aux = nullptr;
     uint32_t *pid = nullptr, *pids = nullptr, *keys = nullptr, *sorted_keys = nullptr, *perm = nullptr,
-             *cell_start = nullptr, *sort_ws = nullptr;
+             *offsets = nullptr, *cell_start = nullptr, *sort_ws = nullptr;
     SortPlan plan{};
     std::vector<DevObject> objs_host;
     DevObject* objs_dev = nullptr;
@@ -95,7 +95,7 @@ struct vfs_sim {
     bool stepped = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int block = 256;
+    vfs_tuning tuning{8, 16, 256, 0};
 
     StepBuffers buffers() const {
         StepBuffers b{};
@@ -105,6 +105,7 @@ struct vfs_sim {
         b.keys = keys;
         b.sorted_keys = sorted_keys;
         b.perm = perm;
+        b.offsets = offsets;
         b.cell_start = cell_start;
         b.xs = xs;
         b.vs = vs;
@@ -126,18 +127,20 @@ struct vfs_sim {
 
     int record_substep(hipStream_t s, StageTimer* timer) {
         const StepBuffers b = buffers();
-        VFS_TRY(sort_keys_iota(plan, keys, sorted_keys, perm, sort_ws, s, timer));
-        if (timer) timer->begin(s, "cell_ranges");
-        VFS_TRY(launch_fill_ranges(sorted_keys, n, cell_start, nullptr, 0, s));
-        if (timer) timer->end(s);
+        // the sort's preparation kernel also resets the offsets table to n
+        VFS_TRY(sort_keys_iota(plan, keys, sorted_keys, perm, sort_ws, s, timer, offsets, (size_t)n + 1, n));
         if (timer) timer->begin(s, "reorder");
-        VFS_TRY(launch_reorder_state(b, s));
+        VFS_TRY(launch_reorder_state(b, s));  // + key starts into the offsets table
+        if (timer) timer->end(s);
+        if (timer) timer->begin(s, "cell_ranges");
+        VFS_TRY(launch_cell_start_scan(plan, offsets, cell_start, sort_ws, s));
         if (timer) timer->end(s);
         if (timer) timer->begin(s, "density");
-        VFS_TRY(launch_density(b, consts, objs_dev, (uint32_t)objs_host.size(), block, s));
+        VFS_TRY(launch_density(b, consts, objs_dev, (uint32_t)objs_host.size(), tuning.block_size,
+                               tuning.stage_rows, s));
         if (timer) timer->end(s);
         if (timer) timer->begin(s, "forces_integrate");
-        VFS_TRY(launch_forces_integrate(b, consts, block, s));
+        VFS_TRY(launch_forces_integrate(b, consts, tuning.block_size, tuning.stage_rows, s));
         if (timer) timer->end(s);
         return 0;
     }
@@ -163,6 +166,46 @@ struct vfs_sim {
         graph = g;
         VFS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         return 0;
+    }
+
+    // Device copy of the substep input (x, v, ids, keys, densities); used by
+    // the autotuner so that every trial starts from the same state.
+    struct Snapshot {
+        float4 *x = nullptr, *v = nullptr;
+        float* rho = nullptr;
+        uint32_t *pid = nullptr, *keys = nullptr;
+        bool keys_valid = false;
+    };
+
+    int snapshot(Snapshot& sn, bool restore) {
+        const size_t n4 = (size_t)n * 16, n1 = (size_t)n * 4;
+        auto cp = [&](void* dst, const void* src, size_t bytes) {
+            return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream);
+        };
+        if (!restore) {
+            if (hipMalloc(&sn.x, n4 ? n4 : 16) != hipSuccess || hipMalloc(&sn.v, n4 ? n4 : 16) != hipSuccess ||
+                hipMalloc(&sn.rho, n1 ? n1 : 4) != hipSuccess || hipMalloc(&sn.keys, n1 ? n1 : 4) != hipSuccess ||
+                (pid && hipMalloc(&sn.pid, n1 ? n1 : 4) != hipSuccess))
+                return fail(VFS_ERR_NOMEM, "snapshot allocation failed");
+            sn.keys_valid = keys_valid;
+            if (n && (cp(sn.x, x, n4) || cp(sn.v, v, n4) || cp(sn.rho, rho, n1) || cp(sn.keys, keys, n1) ||
+                      (pid && cp(sn.pid, pid, n1))))
+                return fail(VFS_ERR_HIP, "snapshot copy failed");
+        } else {
+            if (n && (cp(x, sn.x, n4) || cp(v, sn.v, n4) || cp(rho, sn.rho, n1) || cp(keys, sn.keys, n1) ||
+                      (pid && cp(pid, sn.pid, n1))))
+                return fail(VFS_ERR_HIP, "snapshot restore failed");
+            keys_valid = sn.keys_valid;
+        }
+        VFS_HIP(hipStreamSynchronize(stream));
+        return 0;
+    }
+
+    static void free_snapshot(Snapshot& sn) {
+        void* ptrs[] = {sn.x, sn.v, sn.rho, sn.pid, sn.keys};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        sn = Snapshot{};
     }
 
     // Makes `waiter` wait for the work queued so far on `signaller`.
@@ -206,9 +249,9 @@ int dev_alloc(T** p, size_t count) {
 void free_sim(vfs_sim* s) {
     if (!s) return;
     s->drop_graph();
-    void* ptrs[] = {s->x,    s->v,    s->xs,   s->vs,          s->bnd,  s->accel,      s->rho,
-                    s->aux,  s->pid,  s->pids, s->keys,        s->sorted_keys, s->perm, s->cell_start,
-                    s->sort_ws, s->objs_dev};
+    void* ptrs[] = {s->x,    s->v,    s->xs,   s->vs,          s->bnd,  s->accel,  s->rho,
+                    s->aux,  s->pid,  s->pids, s->keys,        s->sorted_keys, s->perm, s->offsets,
+                    s->cell_start, s->sort_ws, s->objs_dev};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (float* g : s->grids)
@@ -357,13 +400,14 @@ int vfs_sim_create(const vfs_params* params, const vfs_boundary_object* objects,
     r |= dev_alloc(&s->keys, n);
     r |= dev_alloc(&s->sorted_keys, n);
     r |= dev_alloc(&s->perm, n);
+    r |= dev_alloc(&s->offsets, (size_t)n + 1);
     r |= dev_alloc(&s->cell_start, (size_t)n + 1);
     if (params->flags & VFS_FLAG_TRACK_IDS) {
         r |= dev_alloc(&s->pid, n);
         r |= dev_alloc(&s->pids, n);
     }
     if (params->flags & VFS_FLAG_STORE_ACCEL) r |= dev_alloc(&s->accel, n);
-    s->plan = make_sort_plan(n, n ? n - 1 : 0, 8, 16);
+    s->plan = make_sort_plan(n, n ? n - 1 : 0, s->tuning.sort_bits, s->tuning.sort_items);
     r |= dev_alloc(&s->sort_ws, s->plan.workspace_words);
     if (r) return bail(VFS_ERR_NOMEM);
     for (uint32_t k = 0; k < n_objects; k++) {
@@ -517,16 +561,7 @@ int vfs_sim_get_hash(vfs_sim* sim, uint32_t* sorted_keys, uint32_t* sorted_indic
     VFS_HIP(hipStreamSynchronize(sim->stream));
     if (sorted_keys && n) VFS_HIP(hipMemcpy(sorted_keys, sim->sorted_keys, (size_t)n * 4, hipMemcpyDeviceToHost));
     if (sorted_indices && n) VFS_HIP(hipMemcpy(sorted_indices, sim->perm, (size_t)n * 4, hipMemcpyDeviceToHost));
-    if (offsets && n) {
-        uint32_t* tmp = nullptr;
-        VFS_TRY(dev_alloc(&tmp, n));
-        int r = launch_offsets_from_ranges(sim->cell_start, n, tmp, sim->stream);
-        if (r == 0 && hipStreamSynchronize(sim->stream) != hipSuccess) r = fail(VFS_ERR_HIP, "sync failed");
-        if (r == 0 && hipMemcpy(offsets, tmp, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess)
-            r = fail(VFS_ERR_HIP, "copy failed");
-        (void)hipFree(tmp);
-        return r;
-    }
+    if (offsets && n) VFS_HIP(hipMemcpy(offsets, sim->offsets, (size_t)n * 4, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -561,11 +596,99 @@ int vfs_sim_device_buffers(vfs_sim* sim, vfs_device_buffers* out) {
     return 0;
 }
 
+int vfs_sim_get_tuning(const vfs_sim* sim, vfs_tuning* tuning) {
+    VFS_CHECK(sim && tuning, VFS_ERR_INVALID, "null argument");
+    *tuning = sim->tuning;
+    return 0;
+}
+
+int vfs_sim_set_tuning(vfs_sim* sim, const vfs_tuning* t) {
+    VFS_CHECK(sim && t, VFS_ERR_INVALID, "null argument");
+    VFS_CHECK(valid_sort_config(t->sort_bits, t->sort_items), VFS_ERR_INVALID,
+              "sort_bits must be 8, 9 or 10 and sort_items 8, 12 or 16");
+    VFS_CHECK(t->block_size == 128 || t->block_size == 256, VFS_ERR_INVALID, "block_size must be 128 or 256");
+    VFS_CHECK(t->stage_rows == 0 || t->stage_rows == 1, VFS_ERR_INVALID, "stage_rows must be 0 or 1");
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    if (t->sort_bits != sim->tuning.sort_bits || t->sort_items != sim->tuning.sort_items) {
+        const SortPlan plan = make_sort_plan(sim->n, sim->n ? sim->n - 1 : 0, t->sort_bits, t->sort_items);
+        uint32_t* ws = nullptr;
+        VFS_TRY(dev_alloc(&ws, plan.workspace_words));
+        (void)hipFree(sim->sort_ws);
+        sim->sort_ws = ws;
+        sim->plan = plan;
+    }
+    sim->tuning = *t;
+    sim->drop_graph();
+    return 0;
+}
+
+int vfs_sim_autotune(vfs_sim* sim, uint32_t substeps_per_trial, vfs_tuning* best, float* best_ms) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    VFS_CHECK(substeps_per_trial >= 1, VFS_ERR_INVALID, "substeps_per_trial must be >= 1");
+    const vfs_tuning start = sim->tuning;
+    vfs_tuning winner = start;
+    float winner_ms = 0.0f;
+    bool have = false;
+    vfs_sim::Snapshot sn;
+    hipEvent_t a = nullptr, b = nullptr;
+    int rc = sim->snapshot(sn, false);
+    if (rc == 0 && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess))
+        rc = fail(VFS_ERR_HIP, "hipEventCreate failed");
+    const bool saved_stepped = sim->stepped;
+    static const int bits[] = {8, 10};
+    static const int items[] = {12, 16};
+    static const int blocks[] = {128, 256};
+    static const int stage[] = {0, 1};
+    for (int sb : bits)
+        for (int si : items)
+            for (int bs : blocks)
+                for (int st : stage) {
+                    if (rc != 0) break;
+                    const vfs_tuning t{sb, si, bs, st};
+                    rc = vfs_sim_set_tuning(sim, &t);
+                    // every trial starts from the same state; one untimed substep
+                    // builds the graph and warms the caches
+                    if (rc == 0) rc = sim->snapshot(sn, true);
+                    if (rc == 0) rc = sim->substeps(1, sim->stream);
+                    if (rc != 0) break;
+                    (void)hipEventRecord(a, sim->stream);
+                    rc = sim->substeps(substeps_per_trial, sim->stream);
+                    (void)hipEventRecord(b, sim->stream);
+                    if (rc == 0 && hipEventSynchronize(b) != hipSuccess) rc = fail(VFS_ERR_HIP, "timing failed");
+                    if (rc != 0) break;
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, a, b);
+                    ms /= (float)substeps_per_trial;
+                    if (!have || ms < winner_ms) {
+                        winner = t;
+                        winner_ms = ms;
+                        have = true;
+                    }
+                }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    // the scene state is left as it was before the trials
+    if (sn.x) {
+        const int r2 = sim->snapshot(sn, true);
+        if (rc == 0) rc = r2;
+    }
+    vfs_sim::free_snapshot(sn);
+    sim->stepped = saved_stepped;
+    if (rc != 0) {
+        (void)vfs_sim_set_tuning(sim, &start);
+        return rc;
+    }
+    VFS_TRY(vfs_sim_set_tuning(sim, &winner));
+    if (best) *best = winner;
+    if (best_ms) *best_ms = winner_ms;
+    return 0;
+}
+
 int vfs_sim_profile_stages(vfs_sim* sim, uint32_t n, float* ms, uint32_t* launches, const char** names,
                            uint32_t max_stages, uint32_t* n_stages) {
     VFS_CHECK(sim && n_stages, VFS_ERR_INVALID, "null argument");
-    static const char* kNames[] = {"sort_prepare", "sort_histogram", "sort_pass", "cell_ranges",
-                                   "reorder",      "density",        "forces_integrate"};
+    static const char* kNames[] = {"sort_prepare", "sort_histogram", "sort_pass", "reorder",
+                                   "cell_ranges",  "density",        "forces_integrate"};
     constexpr uint32_t kN = sizeof(kNames) / sizeof(kNames[0]);
     *n_stages = 0;
     if (sim->n == 0 || n == 0) return 0;

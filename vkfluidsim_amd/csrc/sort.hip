@@ -237,49 +237,124 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
     }
 }
 
-// Cell ranges from sorted keys: cell_start[k] = first i with sorted[i] >= k,
-// for k in [0, n] (so bucket k is [cell_start[k], cell_start[k+1])), and/or
-// the reference offsets table (first index of key k, n if empty; init=0 only
-// writes the starts of present keys).  Thread i owns keys (sorted[i-1], sorted[i]];
-// a wave fills long runs cooperatively.
-__global__ __launch_bounds__(256) void k_fill_ranges(const uint32_t* __restrict__ sk, uint32_t n,
-                                                     uint32_t* __restrict__ cell_start,
-                                                     uint32_t* __restrict__ offsets, int init) {
+// Reference offsets table from sorted keys: T[k] = first i with sorted[i] == k
+// (CalculateOffsets, spatial_offsets.slang:19-31).  T must hold the empty
+// sentinel for absent keys beforehand (InitOffsets) when a full table is wanted.
+__global__ __launch_bounds__(256) void k_key_starts(const uint32_t* __restrict__ sk, uint32_t n,
+                                                    uint32_t* __restrict__ table) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    uint32_t lo = 1, hi = 0;  // empty
-    if (i <= n) {
-        lo = (i == 0) ? 0u : sk[i - 1] + 1u;
-        hi = (i == n) ? n : sk[i];
-        if (hi > n) hi = n;
+    if (i >= n) return;
+    const uint32_t k = sk[i];
+    if ((i == 0 || sk[i - 1] != k) && k < n) table[k] = i;
+}
+
+// Exclusive min over one value per thread of a 256-thread block (identity ~0u).
+__device__ __forceinline__ uint32_t block_exclusive_min(uint32_t v, uint32_t* s_tmp, uint32_t& total) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x = min(x, y);
     }
-    bool has = lo <= hi;
-    if (offsets && !init) {
-        if (has && i < n && hi < n) offsets[hi] = i;
-        if (!cell_start) return;
+    const uint32_t ex_wave = __shfl_up(x, 1, 64);
+    if (lane == 63) s_tmp[w] = x;
+    __syncthreads();
+    uint32_t wave_off = 0xFFFFFFFFu, tot = 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < kWaves; i++) {
+        const uint32_t t = s_tmp[i];
+        if (i < w) wave_off = min(wave_off, t);
+        tot = min(tot, t);
     }
-    while (true) {
-        const uint64_t m = __ballot(has);
-        if (m == 0) break;
-        const int leader = __ffsll((unsigned long long)m) - 1;
-        const uint32_t L = __shfl(lo, leader, 64);
-        const uint32_t H = __shfl(hi, leader, 64);
-        const uint32_t V = __shfl(i, leader, 64);
-        for (uint32_t k = L + (uint32_t)lane; k <= H; k += 64u) {
-            if (cell_start) cell_start[k] = V;
-            if (offsets && init && k < n) offsets[k] = (k == H && V < n) ? V : n;
+    __syncthreads();
+    total = tot;
+    return lane == 0 ? wave_off : min(wave_off, ex_wave);
+}
+
+// Cell ranges from the offsets table: cell_start[k] = min(T[k..m-1]) over the
+// m = n + 1 entries (T[n] = n), i.e. the start of the next present key, so
+// bucket k is [cell_start[k], cell_start[k+1]).  A reverse inclusive min-scan
+// in one pass: tiles are claimed from the top of the table down and resolve
+// their prefix by decoupled look-back over the tiles above them.
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_cell_start_scan(const uint32_t* __restrict__ table, uint32_t m,
+                                                         uint32_t* __restrict__ cell_start,
+                                                         uint32_t* __restrict__ status,
+                                                         uint32_t* __restrict__ tile_counter) {
+    constexpr int TILE = kThreads * ITEMS;
+    __shared__ uint32_t s_data[TILE];
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_prefix;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const int64_t top = (int64_t)m - (int64_t)tile * TILE;  // exclusive upper index of this tile
+    // element e of the tile is table index top - 1 - e
+    for (int e = tid; e < TILE; e += kThreads) {
+        const int64_t idx = top - 1 - e;
+        s_data[e] = idx >= 0 ? table[idx] : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    uint32_t local[ITEMS];
+    uint32_t run = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        run = min(run, s_data[tid * ITEMS + j]);
+        local[j] = run;
+    }
+    uint32_t total;
+    const uint32_t excl = block_exclusive_min(run, s_tmp, total);
+    if (tid < 64) {
+        // Wave-parallel look-back: lane l inspects tile (base - l); a window
+        // resolves once every tile up to the nearest inclusive one is ready.
+        const int lane = tid;
+        const uint32_t tv = total & kValueMask;
+        if (lane == 0) st_release_agent(&status[tile], (tile == 0 ? kFlagInclusive : kFlagAggregate) | tv);
+        uint32_t prefix = kValueMask;
+        if (tile > 0) {
+            int64_t base = (int64_t)tile - 1;
+            while (true) {
+                const int64_t t = base - lane;
+                const uint32_t sv = t >= 0 ? ld_acquire_agent(&status[t]) : (kFlagInclusive | kValueMask);
+                const uint32_t flag = sv & ~kValueMask;
+                const uint64_t incl = __ballot(flag == kFlagInclusive);
+                const uint64_t wait = __ballot(flag == 0);
+                const int first = incl ? __ffsll((unsigned long long)incl) - 1 : 64;
+                const uint64_t upto = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+                if (wait & upto) continue;
+                uint32_t v = lane <= first ? (sv & kValueMask) : kValueMask;
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o, 64));
+                prefix = min(prefix, v);
+                if (first < 64) break;
+                base -= 64;
+            }
+            if (lane == 0) st_release_agent(&status[tile], kFlagInclusive | min(prefix, tv));
         }
-        if (lane == leader) has = false;
+        if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    const uint32_t pre = min(s_prefix, excl);
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) s_data[tid * ITEMS + j] = min(pre, local[j]);
+    __syncthreads();
+    for (int e = tid; e < TILE; e += kThreads) {
+        const int64_t idx = top - 1 - e;
+        if (idx >= 0) cell_start[idx] = s_data[e];
     }
 }
 
-// Reference offsets from cell ranges (empty bucket -> n).
-__global__ __launch_bounds__(256) void k_offsets_from_ranges(const uint32_t* __restrict__ cs, uint32_t n,
-                                                             uint32_t* __restrict__ offsets) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t a = cs[k], b = cs[k + 1];
-    offsets[k] = a < b ? a : n;
+// Zeroes one region and fills another with a value (per-substep preparation).
+__global__ __launch_bounds__(256) void k_prepare(uint32_t* __restrict__ zero, size_t zero_words,
+                                                 uint32_t* __restrict__ fill, size_t fill_words,
+                                                 uint32_t fill_value) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < zero_words; i += stride) zero[i] = 0u;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < fill_words; i += stride)
+        fill[i] = fill_value;
 }
 
 __global__ __launch_bounds__(256) void k_reorder_f32x3(const uint32_t* __restrict__ perm,
@@ -409,6 +484,10 @@ SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int i
     p.off_hist = take(kMaxPasses * radix);
     p.off_counters = take(kMaxPasses);
     p.off_status = take((size_t)p.passes * p.num_tiles * radix);
+    const uint32_t cs_tile = (uint32_t)kThreads * kCellStartItems;
+    p.cs_tiles = (uint32_t)(((uint64_t)n + 1 + cs_tile - 1) / cs_tile);
+    p.off_cs_counter = take(1);
+    p.off_cs_status = take(p.cs_tiles);
     p.zero_begin = p.off_hist;
     p.workspace_words = off;
     return p;
@@ -439,7 +518,8 @@ static void launch_pass_items(int items, const uint32_t* ki, const uint32_t* vi,
 }
 
 int sort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
-                   uint32_t* workspace, hipStream_t s, StageTimer* timer) {
+                   uint32_t* workspace, hipStream_t s, StageTimer* timer, uint32_t* fill, size_t fill_words,
+                   uint32_t fill_value) {
     const uint32_t n = p.n;
     if (n == 0) return 0;
     const size_t radix = (size_t)1 << p.bits;
@@ -448,8 +528,9 @@ int sort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_ou
     uint32_t* status = workspace + p.off_status;
     const size_t zero_words = p.workspace_words - p.zero_begin;
     if (timer) timer->begin(s, "sort_prepare");
-    hipLaunchKernelGGL(k_zero_words, dim3(std::min<uint32_t>(grid_for(zero_words), 2048u)), dim3(256), 0, s,
-                       workspace + p.zero_begin, zero_words);
+    hipLaunchKernelGGL(k_prepare, dim3(std::min<uint32_t>(grid_for(std::max(zero_words, fill_words)), 2048u)),
+                       dim3(256), 0, s, workspace + p.zero_begin, zero_words, fill, fill ? fill_words : 0,
+                       fill_value);
     VFS_TRY(check_launch("sort_prepare"));
     if (timer) timer->end(s);
     if (timer) timer->begin(s, "sort_histogram");
@@ -495,17 +576,25 @@ int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, 
     return check_launch("hash_keys");
 }
 
-int launch_fill_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t* cell_start, uint32_t* offsets, int init,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_fill_ranges, dim3(grid_for((uint64_t)n + 1)), dim3(256), 0, s, sorted_keys, n, cell_start,
-                       offsets, init);
-    return check_launch("fill_ranges");
+int launch_key_starts(const uint32_t* sorted_keys, uint32_t n, uint32_t* table, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_key_starts, dim3(grid_for(n)), dim3(256), 0, s, sorted_keys, n, table);
+    return check_launch("key_starts");
 }
 
-int launch_offsets_from_ranges(const uint32_t* cell_start, uint32_t n, uint32_t* offsets, hipStream_t s) {
-    if (n == 0) return 0;
-    hipLaunchKernelGGL(k_offsets_from_ranges, dim3(grid_for(n)), dim3(256), 0, s, cell_start, n, offsets);
-    return check_launch("offsets_from_ranges");
+int launch_fill_value(uint32_t* a, size_t count, uint32_t value, hipStream_t s) {
+    if (count == 0) return 0;
+    hipLaunchKernelGGL(k_prepare, dim3(std::min<uint32_t>(grid_for(count), 2048u)), dim3(256), 0, s, nullptr,
+                       (size_t)0, a, count, value);
+    return check_launch("fill_value");
+}
+
+int launch_cell_start_scan(const SortPlan& p, const uint32_t* table, uint32_t* cell_start, uint32_t* workspace,
+                           hipStream_t s) {
+    if (p.cs_tiles == 0) return 0;
+    hipLaunchKernelGGL(k_cell_start_scan<kCellStartItems>, dim3(p.cs_tiles), dim3(256), 0, s, table, p.n + 1,
+                       cell_start, workspace + p.off_cs_status, workspace + p.off_cs_counter);
+    return check_launch("cell_start_scan");
 }
 
 int launch_reorder_f32x3(const uint32_t* perm, const float* src, float* dst, uint32_t n, hipStream_t s) {

@@ -11,6 +11,7 @@
 namespace vfs {
 
 constexpr int kMaxPasses = 4;
+constexpr int kCellStartItems = 8;
 
 // Optional per-stage event timing (used by vfs_sim_profile_stages).
 struct StageTimer {
@@ -28,6 +29,8 @@ struct SortPlan {
     uint32_t num_tiles;
     size_t off_tmp[4];
     size_t off_hist, off_counters, off_status;
+    uint32_t cs_tiles;                  // tiles of the cell-start scan over n + 1 entries
+    size_t off_cs_counter, off_cs_status;
     size_t zero_begin;
     size_t workspace_words;
 };
@@ -37,14 +40,20 @@ int valid_sort_config(int bits, int items);
 
 // Stable sort of (keys_in, iota): keys_out = sorted keys, perm_out[i] = input
 // index of the i-th smallest (key, index).  keys_in is not modified.
+// The preparation kernel of the sort also fills fill[0..fill_words) with
+// fill_value when fill is not null (the offsets table of the substep).
 int sort_keys_iota(const SortPlan& plan, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
-                   uint32_t* workspace, hipStream_t s, StageTimer* timer);
+                   uint32_t* workspace, hipStream_t s, StageTimer* timer, uint32_t* fill = nullptr,
+                   size_t fill_words = 0, uint32_t fill_value = 0);
 
 int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,
                      uint32_t* keys, hipStream_t s);
-int launch_fill_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t* cell_start, uint32_t* offsets,
-                       int init, hipStream_t s);
-int launch_offsets_from_ranges(const uint32_t* cell_start, uint32_t n, uint32_t* offsets, hipStream_t s);
+// table[sorted_keys[i]] = i where a key starts (CalculateOffsets).
+int launch_key_starts(const uint32_t* sorted_keys, uint32_t n, uint32_t* table, hipStream_t s);
+int launch_fill_value(uint32_t* a, size_t count, uint32_t value, hipStream_t s);
+// cell_start[k] = min(table[k..n]) for k in [0, n] (table has n + 1 entries, table[n] = n).
+int launch_cell_start_scan(const SortPlan& plan, const uint32_t* table, uint32_t* cell_start,
+                           uint32_t* workspace, hipStream_t s);
 int launch_reorder_f32x3(const uint32_t* perm, const float* src, float* dst, uint32_t n, hipStream_t s);
 int launch_iota(uint32_t* a, uint32_t n, hipStream_t s);
 int launch_copy_words(const uint32_t* src, uint32_t* dst, uint32_t n, hipStream_t s);
@@ -61,6 +70,7 @@ struct StepBuffers {
     // sorted / reordered
     uint32_t* sorted_keys;
     uint32_t* perm;
+    uint32_t* offsets;     // n + 1: reference offsets table (n for empty keys), entry n = n
     uint32_t* cell_start;  // n + 1
     float4* xs;
     float4* vs;
@@ -74,9 +84,10 @@ struct StepBuffers {
 };
 
 int launch_reorder_state(const StepBuffers& b, hipStream_t s);
+// block: 128 or 256 threads; staged: stage stencil rows of positions in LDS.
 int launch_density(const StepBuffers& b, const SimConsts& c, const DevObject* objs, uint32_t n_obj, int block,
-                   hipStream_t s);
-int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, int block, hipStream_t s);
+                   int staged, hipStream_t s);
+int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, int block, int staged, hipStream_t s);
 
 // Volume-map/SDF grid generator for analytic shapes (FP64 on the device).
 int launch_volume_map(int kind, const double* shape_params, const uint32_t* res, double support_radius,

@@ -179,64 +179,202 @@ __device__ void boundary_volume(const DevObject* __restrict__ objs, uint32_t n_o
 
 // ------------------------------------------------------------------ kernels
 
-__global__ __launch_bounds__(256) void k_reorder_state(StepBuffers b) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// The hardware deals workgroups to the 8 XCDs round robin; each XCD has its own
+// L2.  Giving XCD x the contiguous block range [x * nb/8, (x+1) * nb/8) keeps a
+// compact slab of sorted particles (and its neighbour cells) in one L2 instead
+// of every L2 streaming the whole particle set.  nblocks is a multiple of 8.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t nblocks) {
+    const uint32_t b = blockIdx.x;
+    return (b & 7u) * (nblocks >> 3) + (b >> 3);
+}
+
+// In-range neighbours are appended to a per-lane LDS queue and evaluated once
+// any active lane holds kQueue of them, so the pair arithmetic runs with most
+// lanes busy instead of with the ~15% of lanes whose candidate happens to be in
+// range.  Queued pairs are evaluated in arrival order: sums keep the reference
+// order.  Candidates are read kUnroll at a time (all loads issued before the
+// tests); every candidate writes the next queue slot and only advances the
+// count when it is in range, so the test has no branch.  A queue can therefore
+// hold up to kQueue - 1 + kUnroll entries before the flush check.
+constexpr int kUnroll = 8;
+constexpr int kQueue = 12;
+constexpr int kQueueCap = kQueue + kUnroll;
+
+// Row (dz, dy) of the 27-cell stencil: the buckets of cells (cx-1..cx+1, cy+dy,
+// cz+dz) as up to three [begin, end) ranges of sorted indices, merged into one
+// when their keys are consecutive (then the buckets are adjacent).
+struct RowRanges {
+    uint32_t b0, e0, b1, e1, b2, e2;
+    int nr;
+};
+
+__device__ __forceinline__ RowRanges row_ranges(int32_t cx, int32_t cy, int32_t cz, int dy, int dz,
+                                                const SimConsts& c, const uint32_t* __restrict__ cs) {
+    const uint32_t k0 = cell_key(cx, cy, cz, -1, dy, dz, c);
+    const uint32_t k1 = cell_key(cx, cy, cz, 0, dy, dz, c);
+    const uint32_t k2 = cell_key(cx, cy, cz, 1, dy, dz, c);
+    RowRanges r;
+    if (k1 == k0 + 1u && k2 == k1 + 1u) {
+        r.b0 = cs[k0];
+        r.e0 = cs[k2 + 1u];
+        r.b1 = r.e1 = r.b2 = r.e2 = 0u;
+        r.nr = 1;
+    } else {
+        r.b0 = cs[k0];
+        r.e0 = cs[k0 + 1u];
+        r.b1 = cs[k1];
+        r.e1 = cs[k1 + 1u];
+        r.b2 = cs[k2];
+        r.e2 = cs[k2 + 1u];
+        r.nr = 3;
+    }
+    return r;
+}
+
+// Optional LDS staging of a block's stencil row: the particles of a block sit in
+// a few consecutive cells, so their row ranges overlap in one short span of
+// sorted indices that the block loads once.  Rows whose span exceeds the stage
+// read global memory.
+constexpr uint32_t kStage = 640;
+
+template <int WAVES>
+struct RowStage {
+    float4 x[kStage];
+    uint32_t lo[WAVES], hi[WAVES];
+};
+
+// Block-wide span [L, H) of the lanes' row ranges, staged when it fits.  Two
+// barriers per row: the span slots of row r+1 are written only after every wave
+// passed the second barrier of row r (its slot reads are done), and the stage
+// of row r+1 only after the first barrier of row r+1 (every wave finished
+// reading the stage of row r).
+template <int WAVES>
+__device__ __forceinline__ bool stage_row(RowStage<WAVES>& st, const RowRanges& r, bool active,
+                                          const float4* __restrict__ xs, uint32_t& L) {
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    if (active) {
+        auto span = [&](uint32_t jb, uint32_t je) {
+            if (je > jb) {
+                lo = min(lo, jb);
+                hi = max(hi, je);
+            }
+        };
+        span(r.b0, r.e0);
+        if (r.nr == 3) {
+            span(r.b1, r.e1);
+            span(r.b2, r.e2);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor(lo, o, 64));
+        hi = max(hi, (uint32_t)__shfl_xor(hi, o, 64));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        st.lo[w] = lo;
+        st.hi[w] = hi;
+    }
+    __syncthreads();
+    uint32_t bl = st.lo[0], bh = st.hi[0];
+#pragma unroll
+    for (int k = 1; k < WAVES; k++) {
+        bl = min(bl, st.lo[k]);
+        bh = max(bh, st.hi[k]);
+    }
+    const bool staged = bh > bl && bh - bl <= kStage;
+    if (staged)
+        for (uint32_t t = threadIdx.x; t < bh - bl; t += 64u * WAVES) st.x[t] = xs[bl + t];
+    __syncthreads();
+    L = bl;
+    return staged;
+}
+
+// Visits candidates jb..je-1 in order with kUnroll loads in flight per lane.
+// hit(j, q, valid) appends j to the queue when valid and in range.
+template <bool FROM_STAGE, class Hit, class Flush>
+__device__ __forceinline__ void walk_range(uint32_t jb, uint32_t je, const float4* __restrict__ src, uint32_t L,
+                                           const int& cnt, Hit&& hit, Flush&& flush) {
+    for (uint32_t j0 = jb; j0 < je; j0 += (uint32_t)kUnroll) {
+        float4 q[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            const uint32_t j = min(j0 + (uint32_t)u, je - 1u);
+            q[u] = FROM_STAGE ? src[j - L] : src[j];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) hit(j0 + (uint32_t)u, q[u], j0 + (uint32_t)u < je);
+        if (__any(cnt >= kQueue)) flush();
+    }
+}
+
+// Reorder (reorder.slang:11-19) of x, v (and ids) by the sort permutation, and
+// CalculateOffsets (spatial_offsets.slang:19-31) into the offsets table.
+__global__ __launch_bounds__(256) void k_reorder_state(StepBuffers b, uint32_t nblocks) {
+    const uint32_t i = xcd_block(nblocks) * 256u + threadIdx.x;
     if (i >= b.n) return;
     const uint32_t s = b.perm[i];
     b.xs[i] = b.x[s];
     b.vs[i] = b.v[s];
     if (b.pid) b.pids[i] = b.pid[s];
+    const uint32_t k = b.sorted_keys[i];
+    if (i == 0 || b.sorted_keys[i - 1] != k) b.offsets[k] = i;
 }
 
 // Boundary volume, CalculateDensities (slang:153-197) and the per-particle
 // EOS terms the force pass reads for every neighbour.
-__global__ __launch_bounds__(256) void k_density(StepBuffers b, SimConsts c, const DevObject* __restrict__ objs,
-                                                 uint32_t n_obj) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    const float4 p4 = b.xs[i];
+template <int BLOCK, bool STAGED>
+__global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, const DevObject* __restrict__ objs,
+                                                   uint32_t n_obj, uint32_t nblocks) {
+    __shared__ float s_r2[kQueueCap][BLOCK];
+    __shared__ RowStage<STAGED ? BLOCK / 64 : 1> st_mem[1];
+    const int tid = threadIdx.x;
+    const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + tid;
+    const bool active = i < c.n;
+    const float4 p4 = b.xs[active ? i : 0u];
     const float pos[3] = {p4.x, p4.y, p4.z};
-    float psi, g[3];
-    boundary_volume(objs, n_obj, pos, c, psi, g);
+    float psi = 0.0f, g[3] = {0.0f, 0.0f, 0.0f};
+    if (active) boundary_volume(objs, n_obj, pos, c, psi, g);
 
     const int32_t cx = floor_to_cell(pos[0] / c.cell);
     const int32_t cy = floor_to_cell(pos[1] / c.cell);
     const int32_t cz = floor_to_cell(pos[2] / c.cell);
     const float4* __restrict__ xs = b.xs;
-    const uint32_t* __restrict__ cs = b.cell_start;
     float density = 0.0f;
+    int cnt = 0;
+    auto hit = [&](uint32_t, const float4& q, bool valid) {
+        const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
+        const float r2 = (ox * ox + oy * oy) + oz * oz;
+        s_r2[cnt][tid] = r2;
+        cnt += (valid && !(r2 > c.h2)) ? 1 : 0;
+    };
+    auto flush = [&]() {
+        for (int t = 0; t < cnt; t++) density += c.mass * kernel_w(sqrtf(s_r2[t][tid]), c);
+        cnt = 0;
+    };
     for (int dz = -1; dz <= 1; dz++) {
         for (int dy = -1; dy <= 1; dy++) {
-            uint32_t k[3];
-#pragma unroll
-            for (int dx = 0; dx < 3; dx++) k[dx] = cell_key(cx, cy, cz, dx - 1, dy, dz, c);
-            uint32_t ranges[6];
-            int nr;
-            if (k[1] == k[0] + 1u && k[2] == k[1] + 1u) {
-                ranges[0] = cs[k[0]];
-                ranges[1] = cs[k[2] + 1u];
-                nr = 1;
-            } else {
-#pragma unroll
-                for (int t = 0; t < 3; t++) {
-                    ranges[2 * t] = cs[k[t]];
-                    ranges[2 * t + 1] = cs[k[t] + 1u];
-                }
-                nr = 3;
-            }
-            for (int rr = 0; rr < nr; rr++) {
-                const uint32_t jb = ranges[2 * rr], je = ranges[2 * rr + 1];
-                for (uint32_t j = jb; j < je; j++) {
-                    const float4 q = xs[j];
-                    const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
-                    const float r2 = (ox * ox + oy * oy) + oz * oz;
-                    if (r2 > c.h2) continue;
-                    const float r = sqrtf(r2);
-                    density += c.mass * kernel_w(r, c);
+            const RowRanges r = row_ranges(cx, cy, cz, dy, dz, c, b.cell_start);
+            uint32_t L = 0;
+            bool staged = false;
+            if (STAGED) staged = stage_row(st_mem[0], r, active, xs, L);
+            if (active) {
+                auto visit = [&](uint32_t jb, uint32_t je) {
+                    if (STAGED && staged)
+                        walk_range<true>(jb, je, st_mem[0].x, L, cnt, hit, flush);
+                    else
+                        walk_range<false>(jb, je, xs, 0u, cnt, hit, flush);
+                };
+                visit(r.b0, r.e0);
+                if (r.nr == 3) {
+                    visit(r.b1, r.e1);
+                    visit(r.b2, r.e2);
                 }
             }
         }
     }
+    if (!active) return;
+    flush();
     density += c.rho0 * psi;
     b.rho[i] = density;
     const float p = eos(density, c);
@@ -246,16 +384,58 @@ __global__ __launch_bounds__(256) void k_density(StepBuffers b, SimConsts c, con
     b.bnd[i] = make_float4(g[0], g[1], g[2], psi);
 }
 
+struct ForceAcc {
+    float pfx, pfy, pfz, vfx, vfy, vfz;
+};
+
+// Pressure (slang:205-262) and viscosity (slang:273-324) terms of one pair.
+__device__ __forceinline__ void force_pair(ForceAcc& f, const float4& p4, const float4& v4, float pdi, uint32_t j,
+                                           const float4* __restrict__ xs, const float4* __restrict__ vs,
+                                           const float2* __restrict__ aux, const SimConsts& c) {
+    const float4 q = xs[j];
+    const float xx = p4.x - q.x, xy = p4.y - q.y, xz = p4.z - q.z;
+    const float r2 = (xx * xx + xy * xy) + xz * xz;
+    const float r = sqrtf(r2);
+    float nx, ny, nz;
+    if (r > 0.0f) {
+        nx = xx / r;
+        ny = xy / r;
+        nz = xz / r;
+    } else {
+        nx = 0.0f;
+        ny = 1.0f;
+        nz = 0.0f;
+    }
+    const float G = kernel_dw(r, c);
+    const float2 aj = aux[j];
+    const float s = (c.mass * (pdi + aj.x)) * G;
+    f.pfx += s * nx;
+    f.pfy += s * ny;
+    f.pfz += s * nz;
+    const float4 vj = vs[j];
+    const float ux = v4.x - vj.x, uy = v4.y - vj.y, uz = v4.z - vj.z;
+    const float sv = aj.y * ((ux * xx + uy * xy) + uz * xz);
+    const float den = r2 + c.visc_eps_h2;
+    f.vfx += (sv * (nx * G)) / den;
+    f.vfy += (sv * (ny * G)) / den;
+    f.vfz += (sv * (nz * G)) / den;
+}
+
 // CalculatePressureAccel (slang:205-271) + CalculateViscousAccel (slang:273-334)
 // in one neighbour walk with separate accumulators, then WallAcceleration and
 // UpdatePositions (slang:336-360), then UpdateSpatialHash of the new position.
-__global__ __launch_bounds__(256) void k_forces_integrate(StepBuffers b, SimConsts c) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
-    const float4 p4 = b.xs[i];
-    const float4 v4 = b.vs[i];
-    const float di = b.rho[i];
-    const float pdi = b.aux[i].x;
+template <int BLOCK, bool STAGED>
+__global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimConsts c, uint32_t nblocks) {
+    __shared__ uint32_t s_j[kQueueCap][BLOCK];
+    __shared__ RowStage<STAGED ? BLOCK / 64 : 1> st_mem[1];
+    const int tid = threadIdx.x;
+    const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + tid;
+    const bool active = i < c.n;
+    const uint32_t ii = active ? i : 0u;
+    const float4 p4 = b.xs[ii];
+    const float4 v4 = b.vs[ii];
+    const float di = b.rho[ii];
+    const float pdi = b.aux[ii].x;
     const float pi = eos(di, c);
 
     const int32_t cx = floor_to_cell(p4.x / c.cell);
@@ -264,75 +444,53 @@ __global__ __launch_bounds__(256) void k_forces_integrate(StepBuffers b, SimCons
     const float4* __restrict__ xs = b.xs;
     const float4* __restrict__ vs = b.vs;
     const float2* __restrict__ aux = b.aux;
-    const uint32_t* __restrict__ cs = b.cell_start;
-    float pfx = 0.0f, pfy = 0.0f, pfz = 0.0f;
-    float vfx = 0.0f, vfy = 0.0f, vfz = 0.0f;
+    ForceAcc f{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int cnt = 0;
+    auto hit = [&](uint32_t j, const float4& q, bool valid) {
+        const float xx = p4.x - q.x, xy = p4.y - q.y, xz = p4.z - q.z;
+        const float r2 = (xx * xx + xy * xy) + xz * xz;
+        s_j[cnt][tid] = j;
+        cnt += (valid && j != i && !(r2 > c.h2)) ? 1 : 0;
+    };
+    auto flush = [&]() {
+        for (int t = 0; t < cnt; t++) force_pair(f, p4, v4, pdi, s_j[t][tid], xs, vs, aux, c);
+        cnt = 0;
+    };
     for (int dz = -1; dz <= 1; dz++) {
         for (int dy = -1; dy <= 1; dy++) {
-            uint32_t k[3];
-#pragma unroll
-            for (int dx = 0; dx < 3; dx++) k[dx] = cell_key(cx, cy, cz, dx - 1, dy, dz, c);
-            uint32_t ranges[6];
-            int nr;
-            if (k[1] == k[0] + 1u && k[2] == k[1] + 1u) {
-                ranges[0] = cs[k[0]];
-                ranges[1] = cs[k[2] + 1u];
-                nr = 1;
-            } else {
-#pragma unroll
-                for (int t = 0; t < 3; t++) {
-                    ranges[2 * t] = cs[k[t]];
-                    ranges[2 * t + 1] = cs[k[t] + 1u];
-                }
-                nr = 3;
-            }
-            for (int rr = 0; rr < nr; rr++) {
-                const uint32_t jb = ranges[2 * rr], je = ranges[2 * rr + 1];
-                for (uint32_t j = jb; j < je; j++) {
-                    if (j == i) continue;
-                    const float4 q = xs[j];
-                    const float xx = p4.x - q.x, xy = p4.y - q.y, xz = p4.z - q.z;
-                    const float r2 = (xx * xx + xy * xy) + xz * xz;
-                    if (r2 > c.h2) continue;
-                    const float r = sqrtf(r2);
-                    float nx, ny, nz;
-                    if (r > 0.0f) {
-                        nx = xx / r;
-                        ny = xy / r;
-                        nz = xz / r;
-                    } else {
-                        nx = 0.0f;
-                        ny = 1.0f;
-                        nz = 0.0f;
-                    }
-                    const float G = kernel_dw(r, c);
-                    const float2 aj = aux[j];
-                    const float s = (c.mass * (pdi + aj.x)) * G;
-                    pfx += s * nx;
-                    pfy += s * ny;
-                    pfz += s * nz;
-                    const float4 vj = vs[j];
-                    const float ux = v4.x - vj.x, uy = v4.y - vj.y, uz = v4.z - vj.z;
-                    const float sv = aj.y * ((ux * xx + uy * xy) + uz * xz);
-                    const float den = r2 + c.visc_eps_h2;
-                    vfx += (sv * (nx * G)) / den;
-                    vfy += (sv * (ny * G)) / den;
-                    vfz += (sv * (nz * G)) / den;
+            const RowRanges r = row_ranges(cx, cy, cz, dy, dz, c, b.cell_start);
+            uint32_t L = 0;
+            bool staged = false;
+            if (STAGED) staged = stage_row(st_mem[0], r, active, xs, L);
+            if (active) {
+                auto visit = [&](uint32_t jb, uint32_t je) {
+                    if (STAGED && staged)
+                        walk_range<true>(jb, je, st_mem[0].x, L, cnt, hit, flush);
+                    else
+                        walk_range<false>(jb, je, xs, 0u, cnt, hit, flush);
+                };
+                visit(r.b0, r.e0);
+                if (r.nr == 3) {
+                    visit(r.b1, r.e1);
+                    visit(r.b2, r.e2);
                 }
             }
         }
     }
+    if (!active) return;
+    flush();
+
     const float pdb = di > 0.0f ? pi / c.rho0_sq : 0.0f;
     const float sb = (pdi + pdb) * c.rho0;
     const float4 bg = b.bnd[i];
-    pfx += sb * bg.x;
-    pfy += sb * bg.y;
-    pfz += sb * bg.z;
+    f.pfx += sb * bg.x;
+    f.pfy += sb * bg.y;
+    f.pfz += sb * bg.z;
 
-    float ax = c.gx + (-pfx), ay = c.gy + (-pfy), az = c.gz + (-pfz);
-    ax += c.nu10 * vfx;
-    ay += c.nu10 * vfy;
-    az += c.nu10 * vfz;
+    float ax = c.gx + (-f.pfx), ay = c.gy + (-f.pfy), az = c.gz + (-f.pfz);
+    ax += c.nu10 * f.vfx;
+    ay += c.nu10 * f.vfy;
+    az += c.nu10 * f.vfz;
     // WallAcceleration
     const float dfx = fminf(p4.x - c.bb_lo[0], 0.0f) - fminf(c.bb_hi[0] - p4.x, 0.0f);
     const float dfy = fminf(p4.y - c.bb_lo[1], 0.0f) - fminf(c.bb_hi[1] - p4.y, 0.0f);
@@ -451,26 +609,52 @@ __global__ __launch_bounds__(128) void k_volume_map(ShapeArgs sh, uint32_t rx, u
 
 inline uint32_t grid_for(uint64_t n, uint32_t block) { return (uint32_t)((n + block - 1) / block); }
 
+// Particle kernels: blocks padded to a multiple of 8 (xcd_block).
+inline uint32_t particle_blocks(uint32_t n, uint32_t block) { return (grid_for(n, block) + 7u) & ~7u; }
+
 }  // namespace
 
 int launch_reorder_state(const StepBuffers& b, hipStream_t s) {
     if (b.n == 0) return 0;
-    hipLaunchKernelGGL(k_reorder_state, dim3(grid_for(b.n, 256)), dim3(256), 0, s, b);
+    const uint32_t nb = particle_blocks(b.n, 256);
+    hipLaunchKernelGGL(k_reorder_state, dim3(nb), dim3(256), 0, s, b, nb);
     return check_launch("reorder_state");
 }
 
 int launch_density(const StepBuffers& b, const SimConsts& c, const DevObject* objs, uint32_t n_obj, int block,
-                   hipStream_t s) {
+                   int staged, hipStream_t s) {
     if (b.n == 0) return 0;
-    (void)block;
-    hipLaunchKernelGGL(k_density, dim3(grid_for(b.n, 256)), dim3(256), 0, s, b, c, objs, n_obj);
+    const uint32_t bs = block == 128 ? 128u : 256u;
+    const uint32_t nb = particle_blocks(b.n, bs);
+    if (bs == 128) {
+        if (staged)
+            hipLaunchKernelGGL((k_density<128, true>), dim3(nb), dim3(128), 0, s, b, c, objs, n_obj, nb);
+        else
+            hipLaunchKernelGGL((k_density<128, false>), dim3(nb), dim3(128), 0, s, b, c, objs, n_obj, nb);
+    } else {
+        if (staged)
+            hipLaunchKernelGGL((k_density<256, true>), dim3(nb), dim3(256), 0, s, b, c, objs, n_obj, nb);
+        else
+            hipLaunchKernelGGL((k_density<256, false>), dim3(nb), dim3(256), 0, s, b, c, objs, n_obj, nb);
+    }
     return check_launch("density");
 }
 
-int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, int block, hipStream_t s) {
+int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, int block, int staged, hipStream_t s) {
     if (b.n == 0) return 0;
-    (void)block;
-    hipLaunchKernelGGL(k_forces_integrate, dim3(grid_for(b.n, 256)), dim3(256), 0, s, b, c);
+    const uint32_t bs = block == 128 ? 128u : 256u;
+    const uint32_t nb = particle_blocks(b.n, bs);
+    if (bs == 128) {
+        if (staged)
+            hipLaunchKernelGGL((k_forces_integrate<128, true>), dim3(nb), dim3(128), 0, s, b, c, nb);
+        else
+            hipLaunchKernelGGL((k_forces_integrate<128, false>), dim3(nb), dim3(128), 0, s, b, c, nb);
+    } else {
+        if (staged)
+            hipLaunchKernelGGL((k_forces_integrate<256, true>), dim3(nb), dim3(256), 0, s, b, c, nb);
+        else
+            hipLaunchKernelGGL((k_forces_integrate<256, false>), dim3(nb), dim3(256), 0, s, b, c, nb);
+    }
     return check_launch("forces_integrate");
 }
 
