@@ -12,6 +12,10 @@
 //   * EOS power: integral exponent in [0,64] -> binary powering in double,
 //     rounded once to the working type; otherwise std::pow in double;
 //   * dot(a,b) = (ax*bx + ay*by) + az*bz; M*v sums columns left to right;
+//   * a vector divided by a scalar (xij / |xij|, the viscous term over
+//     r^2 + 0.01h^2, the boundary normal, the SDF gradient) is multiplied by
+//     the IEEE reciprocal of the scalar; scalar quotients are IEEE divisions
+//     (SPIR-V OpFDiv itself is only specified to 2.5 ulp under Vulkan);
 //   * the 1e-9 / 0.01 / 0.125 / 0.8 literals are taken as float literals;
 //   * float->int cell conversion saturates and maps NaN to 0;
 //   * SampleGrid clamps the base cell to [0, n-2] (the reference reads out of
@@ -284,8 +288,8 @@ static void gradient_vector(const typename Model<R>::Obj& o, R h, const R* x, R*
         to_uvw<R>(o.box_pos, o.box_size, xm, uvw);
         v0[a] = sample_grid<R>(o.sdf.data(), o.res, uvw);
     }
-    const R den = R(2) * eps;
-    for (int a = 0; a < 3; a++) out[a] = (v1[a] - v0[a]) / den;
+    const R inv = R(1) / (R(2) * eps);
+    for (int a = 0; a < 3; a++) out[a] = (v1[a] - v0[a]) * inv;
 }
 
 // mul(M, float4(v, w)).xyz for a column-major 4x4 matrix.
@@ -319,7 +323,8 @@ static void boundary_volume(const Model<R>& m, const R* pos, R& psi, R* grad) {
         if (!(modn > R(1e-9f))) continue;
         R rn[3];
         mat_vec<R>(o.Rm, nrm, R(1), rn);
-        for (int a = 0; a < 3; a++) rn[a] = rn[a] / modn;
+        const R inv_modn = R(1) / modn;
+        for (int a = 0; a < 3; a++) rn[a] = rn[a] * inv_modn;
         const R d = std::fmax(dist + R(0.125f) * m.h, R(0.5f) * m.h);
         R xij[3];
         for (int a = 0; a < 3; a++) {
@@ -328,7 +333,8 @@ static void boundary_volume(const Model<R>& m, const R* pos, R& psi, R* grad) {
         }
         const R xm = std::sqrt(dot3<R>(xij, xij));
         R xn[3];
-        for (int a = 0; a < 3; a++) xn[a] = xm > R(0) ? xij[a] / xm : R(0);
+        const R inv_xm = xm > R(0) ? R(1) / xm : R(0);
+        for (int a = 0; a < 3; a++) xn[a] = xm > R(0) ? xij[a] * inv_xm : R(0);
         psi += volume * m.W(xm);
         const R vg = volume * m.dW(xm);
         for (int a = 0; a < 3; a++) grad[a] += vg * xn[a];
@@ -451,7 +457,8 @@ struct Sim {
                     const R r = std::sqrt(r2);
                     R xn[3];
                     if (r > R(0)) {
-                        for (int c = 0; c < 3; c++) xn[c] = xij[c] / r;
+                        const R inv_r = R(1) / r;
+                        for (int c = 0; c < 3; c++) xn[c] = xij[c] * inv_r;
                     } else {
                         xn[0] = R(0);
                         xn[1] = R(1);
@@ -493,7 +500,8 @@ struct Sim {
                     const R r = std::sqrt(r2);
                     R xn[3];
                     if (r > R(0)) {
-                        for (int c = 0; c < 3; c++) xn[c] = xij[c] / r;
+                        const R inv_r = R(1) / r;
+                        for (int c = 0; c < 3; c++) xn[c] = xij[c] * inv_r;
                     } else {
                         xn[0] = R(0);
                         xn[1] = R(1);
@@ -504,8 +512,8 @@ struct Sim {
                     const R rhoj = std::fmax(rho[j], R(1e-6f));
                     const R G = m.dW(r);
                     const R s = (m.mass / rhoj) * dot3<R>(vij, xij);
-                    const R den = r2 + R(0.01f) * m.h2;
-                    for (int c = 0; c < 3; c++) vf[c] += (s * (xn[c] * G)) / den;
+                    const R inv_den = R(1) / (r2 + R(0.01f) * m.h2);
+                    for (int c = 0; c < 3; c++) vf[c] += (s * (xn[c] * G)) * inv_den;
                 }
             }
             const R f = (R(2) * (R(3) + R(2))) * m.nu;

@@ -66,6 +66,7 @@ SimConsts make_consts(const vfs_params& p) {
         c.bb_hi[a] = p.bbox_pos[a] + p.bbox_size[a];
     }
     c.n = (uint32_t)p.n_particles;
+    c.nbr_cap = kNeighbourCap;
     c.fm = make_fastmod(c.n);
     return c;
 }
@@ -84,6 +85,7 @@ struct vfs_sim {
     float4 *x = nullptr, *v = nullptr, *xs = nullptr, *vs = nullptr, *bnd = nullptr, *accel = nullptr;
     float* rho = nullptr;
     float2* aux = nullptr;
+    uint32_t *nbr = nullptr, *nbr_count = nullptr;
     uint32_t *pid = nullptr, *pids = nullptr, *keys = nullptr, *sorted_keys = nullptr, *perm = nullptr,
              *offsets = nullptr, *cell_start = nullptr, *sort_ws = nullptr;
     SortPlan plan{};
@@ -114,6 +116,8 @@ struct vfs_sim {
         b.aux = aux;
         b.bnd = bnd;
         b.accel = accel;
+        b.nbr = nbr;
+        b.nbr_count = nbr_count;
         b.n = n;
         return b;
     }
@@ -251,7 +255,7 @@ void free_sim(vfs_sim* s) {
     s->drop_graph();
     void* ptrs[] = {s->x,    s->v,    s->xs,   s->vs,          s->bnd,  s->accel,  s->rho,
                     s->aux,  s->pid,  s->pids, s->keys,        s->sorted_keys, s->perm, s->offsets,
-                    s->cell_start, s->sort_ws, s->objs_dev};
+                    s->cell_start, s->sort_ws, s->objs_dev, s->nbr, s->nbr_count};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (float* g : s->grids)
@@ -262,8 +266,8 @@ void free_sim(vfs_sim* s) {
 
 int validate_params(const vfs_params* p) {
     VFS_CHECK(p, VFS_ERR_INVALID, "params is null");
-    VFS_CHECK(p->n_particles >= 0 && p->n_particles < (1 << 30), VFS_ERR_INVALID,
-              "n_particles must be in [0, 2^30)");
+    VFS_CHECK(p->n_particles >= 0 && p->n_particles < (1 << 28), VFS_ERR_INVALID,
+              "n_particles must be in [0, 2^28)");
     VFS_CHECK(p->iterations >= 1, VFS_ERR_INVALID, "iterations must be >= 1");
     VFS_CHECK(p->smooth_radius > 0.0f && std::isfinite(p->smooth_radius), VFS_ERR_INVALID,
               "smooth_radius must be positive");
@@ -401,6 +405,8 @@ int vfs_sim_create(const vfs_params* params, const vfs_boundary_object* objects,
     r |= dev_alloc(&s->sorted_keys, n);
     r |= dev_alloc(&s->perm, n);
     r |= dev_alloc(&s->offsets, (size_t)n + 1);
+    r |= dev_alloc(&s->nbr, (size_t)n * kNeighbourCap);
+    r |= dev_alloc(&s->nbr_count, n);
     r |= dev_alloc(&s->cell_start, (size_t)n + 1);
     if (params->flags & VFS_FLAG_TRACK_IDS) {
         r |= dev_alloc(&s->pid, n);

@@ -80,6 +80,8 @@ struct StepBuffers {
     float2* aux;         // (p / rho^2, mass / max(rho, 1e-6))
     float4* bnd;         // (grad psi xyz, psi)
     float4* accel;       // nullable
+    uint32_t* nbr;       // nbr_cap * n: neighbour lists of the substep, slot-major
+    uint32_t* nbr_count; // n: list lengths (> nbr_cap: overflowed)
     uint32_t n;
 };
 

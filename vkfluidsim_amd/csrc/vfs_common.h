@@ -92,8 +92,12 @@ struct SimConsts {
     float rho0_sq;         // rho0 * rho0
     float bb_lo[3], bb_hi[3];
     uint32_t n;
+    uint32_t nbr_cap;      // neighbour-list slots per particle
     FastMod fm;            // % n
 };
+
+// Neighbour-list slots per particle (the lattice workloads have ~30 neighbours).
+constexpr uint32_t kNeighbourCap = 64;
 
 // Boundary object as the kernels see it (column-major matrices).
 struct DevObject {

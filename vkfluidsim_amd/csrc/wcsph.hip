@@ -7,7 +7,9 @@
 // assets/shaders/simulation/wcsph_with_boundary_model.slang and
 // kernels/kernels_3d.slang) operation by operation; the library is compiled
 // with -ffp-contract=off so that every float operation rounds once, as in the
-// CPU oracle.  Neighbour enumeration visits the 27 hashed cells in the
+// CPU oracle; a vector divided by a scalar is multiplied by the IEEE
+// reciprocal of the scalar, scalar quotients are IEEE divisions (the oracle's
+// canonical reading of the shader arithmetic).  Neighbour enumeration visits the 27 hashed cells in the
 // reference order (z outer, y, x inner; spatial_hash_3d.slang:3-10) and each
 // bucket in ascending sorted index, so sums are accumulated in the reference
 // order.  When the three x-adjacent cells of a row hash to consecutive keys
@@ -154,14 +156,15 @@ __device__ void boundary_volume(const DevObject* __restrict__ objs, uint32_t n_o
             const float v1 = sample_grid(o.sdf, o.res, u);
             to_uvw(o, xm, u);
             const float v0 = sample_grid(o.sdf, o.res, u);
-            nrm[a] = (v1 - v0) / (2.0f * eps);
+            nrm[a] = (v1 - v0) * (1.0f / (2.0f * eps));
         }
         const float modn = sqrtf((nrm[0] * nrm[0] + nrm[1] * nrm[1]) + nrm[2] * nrm[2]);
         if (!(modn > 1e-9f)) continue;
         float rn[3];
         mat_vec(o.Rm, nrm, rn);
+        const float inv_modn = 1.0f / modn;
 #pragma unroll
-        for (int a = 0; a < 3; a++) rn[a] = rn[a] / modn;
+        for (int a = 0; a < 3; a++) rn[a] = rn[a] * inv_modn;
         const float d = fmaxf(dist + 0.125f * c.h, 0.5f * c.h);
         float xij[3];
 #pragma unroll
@@ -172,8 +175,9 @@ __device__ void boundary_volume(const DevObject* __restrict__ objs, uint32_t n_o
         const float xm = sqrtf((xij[0] * xij[0] + xij[1] * xij[1]) + xij[2] * xij[2]);
         psi += volume * kernel_w(xm, c);
         const float vg = volume * kernel_dw(xm, c);
+        const float inv_xm = xm > 0.0f ? 1.0f / xm : 0.0f;
 #pragma unroll
-        for (int a = 0; a < 3; a++) grad[a] += vg * (xm > 0.0f ? xij[a] / xm : 0.0f);
+        for (int a = 0; a < 3; a++) grad[a] += vg * (xm > 0.0f ? xij[a] * inv_xm : 0.0f);
     }
 }
 
@@ -188,6 +192,29 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t nblocks) {
     return (b & 7u) * (nblocks >> 3) + (b >> 3);
 }
 
+// Raw buffer resources over the particle arrays: 32-bit offsets and hardware
+// range checks (a load past num_records bytes returns zeros), so candidate
+// groups need neither 64-bit address arithmetic nor index clamping.
+constexpr int kBufferDword3 = 0x00020000;  // raw (untyped) buffer access on gfx9
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufferDword3);
+}
+
+__device__ __forceinline__ float4 load_f4(__amdgpu_buffer_rsrc_t r, uint32_t index) {
+    const auto raw = __builtin_amdgcn_raw_buffer_load_b128(r, (int)(index * 16u), 0, 0);
+    float4 v;
+    __builtin_memcpy(&v, &raw, sizeof(v));
+    return v;
+}
+
+__device__ __forceinline__ float2 load_f2(__amdgpu_buffer_rsrc_t r, uint32_t index) {
+    const auto raw = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(index * 8u), 0, 0);
+    float2 v;
+    __builtin_memcpy(&v, &raw, sizeof(v));
+    return v;
+}
+
 // In-range neighbours are appended to a per-lane LDS queue and evaluated once
 // any active lane holds kQueue of them, so the pair arithmetic runs with most
 // lanes busy instead of with the ~15% of lanes whose candidate happens to be in
@@ -197,7 +224,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t nblocks) {
 // count when it is in range, so the test has no branch.  A queue can therefore
 // hold up to kQueue - 1 + kUnroll entries before the flush check.
 constexpr int kUnroll = 8;
-constexpr int kQueue = 12;
+constexpr int kQueue = 16;
 constexpr int kQueueCap = kQueue + kUnroll;
 
 // Row (dz, dy) of the 27-cell stencil: the buckets of cells (cx-1..cx+1, cy+dy,
@@ -290,22 +317,30 @@ __device__ __forceinline__ bool stage_row(RowStage<WAVES>& st, const RowRanges& 
     return staged;
 }
 
-// Visits candidates jb..je-1 in order with kUnroll loads in flight per lane.
+// Visits candidates jb..je-1 in order with kUnroll loads in flight per lane:
+// whole groups first (every candidate valid), then one masked tail group whose
+// extra loads read neighbouring particles (or zeros past the array).
 // hit(j, q, valid) appends j to the queue when valid and in range.
-template <bool FROM_STAGE, class Hit, class Flush>
-__device__ __forceinline__ void walk_range(uint32_t jb, uint32_t je, const float4* __restrict__ src, uint32_t L,
-                                           const int& cnt, Hit&& hit, Flush&& flush) {
-    for (uint32_t j0 = jb; j0 < je; j0 += (uint32_t)kUnroll) {
+template <class Load, class Hit, class Flush>
+__device__ __forceinline__ void walk_range(uint32_t jb, uint32_t je, Load&& load, const int& cnt, Hit&& hit,
+                                           Flush&& flush) {
+    uint32_t j0 = jb;
+    for (; j0 + (uint32_t)kUnroll <= je; j0 += (uint32_t)kUnroll) {
         float4 q[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const uint32_t j = min(j0 + (uint32_t)u, je - 1u);
-            q[u] = FROM_STAGE ? src[j - L] : src[j];
-        }
+        for (int u = 0; u < kUnroll; u++) q[u] = load(j0 + (uint32_t)u);
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) hit(j0 + (uint32_t)u, q[u], j0 + (uint32_t)u < je);
+        for (int u = 0; u < kUnroll; u++) hit(j0 + (uint32_t)u, q[u], true);
         if (__any(cnt >= kQueue)) flush();
     }
+    if (j0 < je) {
+        float4 q[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) q[u] = load(j0 + (uint32_t)u);
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) hit(j0 + (uint32_t)u, q[u], j0 + (uint32_t)u < je);
+    }
+    if (__any(cnt >= kQueue)) flush();
 }
 
 // Reorder (reorder.slang:11-19) of x, v (and ids) by the sort permutation, and
@@ -321,12 +356,16 @@ __global__ __launch_bounds__(256) void k_reorder_state(StepBuffers b, uint32_t n
     if (i == 0 || b.sorted_keys[i - 1] != k) b.offsets[k] = i;
 }
 
-// Boundary volume, CalculateDensities (slang:153-197) and the per-particle
-// EOS terms the force pass reads for every neighbour.
+// Boundary volume, CalculateDensities (slang:153-197), the per-particle EOS
+// terms the force pass reads for every neighbour, and this substep's neighbour
+// list: the in-range neighbours j != i in reference order, stored slot-major
+// (nbr[t * n + i]) so that the force pass reads it coalesced.  A particle with
+// more than nbr_cap neighbours stores its full count and the force pass walks
+// its cells again.
 template <int BLOCK, bool STAGED>
 __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, const DevObject* __restrict__ objs,
                                                    uint32_t n_obj, uint32_t nblocks) {
-    __shared__ float s_r2[kQueueCap][BLOCK];
+    __shared__ uint32_t s_j[kQueueCap][BLOCK];
     __shared__ RowStage<STAGED ? BLOCK / 64 : 1> st_mem[1];
     const int tid = threadIdx.x;
     const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + tid;
@@ -340,16 +379,38 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, c
     const int32_t cy = floor_to_cell(pos[1] / c.cell);
     const int32_t cz = floor_to_cell(pos[2] / c.cell);
     const float4* __restrict__ xs = b.xs;
+    const auto xr = make_rsrc(xs, c.n * 16u);
+    uint32_t* __restrict__ list = b.nbr;
     float density = 0.0f;
     int cnt = 0;
-    auto hit = [&](uint32_t, const float4& q, bool valid) {
+    uint32_t listed = 0;
+    auto hit = [&](uint32_t j, const float4& q, bool valid) {
         const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
         const float r2 = (ox * ox + oy * oy) + oz * oz;
-        s_r2[cnt][tid] = r2;
+        s_j[cnt][tid] = j;
         cnt += (valid && !(r2 > c.h2)) ? 1 : 0;
     };
+    // Queued neighbours in order, the next one's position in flight.  The slot
+    // after the last queued one may hold any index; the buffer range check
+    // makes its load harmless.
     auto flush = [&]() {
-        for (int t = 0; t < cnt; t++) density += c.mass * kernel_w(sqrtf(s_r2[t][tid]), c);
+        if (cnt > 0) {
+            uint32_t j = s_j[0][tid];
+            float4 q = load_f4(xr, j);
+            for (int t = 0; t < cnt; t++) {
+                const uint32_t jn = s_j[min(t + 1, kQueueCap - 1)][tid];
+                const float4 qn = load_f4(xr, jn);
+                const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
+                const float r2 = (ox * ox + oy * oy) + oz * oz;
+                density += c.mass * kernel_w(sqrtf(r2), c);
+                if (j != i) {
+                    if (listed < c.nbr_cap) list[(size_t)listed * c.n + i] = j;
+                    listed++;
+                }
+                j = jn;
+                q = qn;
+            }
+        }
         cnt = 0;
     };
     for (int dz = -1; dz <= 1; dz++) {
@@ -361,9 +422,11 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, c
             if (active) {
                 auto visit = [&](uint32_t jb, uint32_t je) {
                     if (STAGED && staged)
-                        walk_range<true>(jb, je, st_mem[0].x, L, cnt, hit, flush);
+                        walk_range(
+                            jb, je, [&](uint32_t j) { return st_mem[0].x[min(j - L, kStage - 1u)]; }, cnt, hit,
+                            flush);
                     else
-                        walk_range<false>(jb, je, xs, 0u, cnt, hit, flush);
+                        walk_range(jb, je, [&](uint32_t j) { return load_f4(xr, j); }, cnt, hit, flush);
                 };
                 visit(r.b0, r.e0);
                 if (r.nr == 3) {
@@ -382,103 +445,104 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, c
     const float mrho = c.mass / fmaxf(density, 1e-6f);
     b.aux[i] = make_float2(pd, mrho);
     b.bnd[i] = make_float4(g[0], g[1], g[2], psi);
+    b.nbr_count[i] = listed;
 }
 
 struct ForceAcc {
     float pfx, pfy, pfz, vfx, vfy, vfz;
 };
 
-// Pressure (slang:205-262) and viscosity (slang:273-324) terms of one pair.
-__device__ __forceinline__ void force_pair(ForceAcc& f, const float4& p4, const float4& v4, float pdi, uint32_t j,
-                                           const float4* __restrict__ xs, const float4* __restrict__ vs,
-                                           const float2* __restrict__ aux, const SimConsts& c) {
-    const float4 q = xs[j];
+// Pressure (slang:205-262) and viscosity (slang:273-324) terms of one pair
+// (neighbour position q, velocity vj, EOS terms aj).
+__device__ __forceinline__ void force_pair(ForceAcc& f, const float4& p4, const float4& v4, float pdi, const float4& q,
+                                           const float4& vj, const float2& aj, const SimConsts& c) {
     const float xx = p4.x - q.x, xy = p4.y - q.y, xz = p4.z - q.z;
     const float r2 = (xx * xx + xy * xy) + xz * xz;
     const float r = sqrtf(r2);
     float nx, ny, nz;
     if (r > 0.0f) {
-        nx = xx / r;
-        ny = xy / r;
-        nz = xz / r;
+        const float inv_r = 1.0f / r;
+        nx = xx * inv_r;
+        ny = xy * inv_r;
+        nz = xz * inv_r;
     } else {
         nx = 0.0f;
         ny = 1.0f;
         nz = 0.0f;
     }
     const float G = kernel_dw(r, c);
-    const float2 aj = aux[j];
     const float s = (c.mass * (pdi + aj.x)) * G;
     f.pfx += s * nx;
     f.pfy += s * ny;
     f.pfz += s * nz;
-    const float4 vj = vs[j];
     const float ux = v4.x - vj.x, uy = v4.y - vj.y, uz = v4.z - vj.z;
     const float sv = aj.y * ((ux * xx + uy * xy) + uz * xz);
-    const float den = r2 + c.visc_eps_h2;
-    f.vfx += (sv * (nx * G)) / den;
-    f.vfy += (sv * (ny * G)) / den;
-    f.vfz += (sv * (nz * G)) / den;
+    const float inv_den = 1.0f / (r2 + c.visc_eps_h2);
+    f.vfx += (sv * (nx * G)) * inv_den;
+    f.vfy += (sv * (ny * G)) * inv_den;
+    f.vfz += (sv * (nz * G)) * inv_den;
 }
 
 // CalculatePressureAccel (slang:205-271) + CalculateViscousAccel (slang:273-334)
-// in one neighbour walk with separate accumulators, then WallAcceleration and
-// UpdatePositions (slang:336-360), then UpdateSpatialHash of the new position.
-template <int BLOCK, bool STAGED>
+// over this substep's neighbour list (one walk, separate accumulators), then
+// WallAcceleration and UpdatePositions (slang:336-360), then UpdateSpatialHash
+// of the new position.  Particles whose list overflowed walk their cells.
+template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimConsts c, uint32_t nblocks) {
-    __shared__ uint32_t s_j[kQueueCap][BLOCK];
-    __shared__ RowStage<STAGED ? BLOCK / 64 : 1> st_mem[1];
-    const int tid = threadIdx.x;
-    const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + tid;
-    const bool active = i < c.n;
-    const uint32_t ii = active ? i : 0u;
-    const float4 p4 = b.xs[ii];
-    const float4 v4 = b.vs[ii];
-    const float di = b.rho[ii];
-    const float pdi = b.aux[ii].x;
+    const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const float4 p4 = b.xs[i];
+    const float4 v4 = b.vs[i];
+    const float di = b.rho[i];
+    const float pdi = b.aux[i].x;
     const float pi = eos(di, c);
-
-    const int32_t cx = floor_to_cell(p4.x / c.cell);
-    const int32_t cy = floor_to_cell(p4.y / c.cell);
-    const int32_t cz = floor_to_cell(p4.z / c.cell);
-    const float4* __restrict__ xs = b.xs;
-    const float4* __restrict__ vs = b.vs;
-    const float2* __restrict__ aux = b.aux;
+    const auto xr = make_rsrc(b.xs, c.n * 16u);
+    const auto vr = make_rsrc(b.vs, c.n * 16u);
+    const auto ar = make_rsrc(b.aux, c.n * 8u);
     ForceAcc f{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    int cnt = 0;
-    auto hit = [&](uint32_t j, const float4& q, bool valid) {
-        const float xx = p4.x - q.x, xy = p4.y - q.y, xz = p4.z - q.z;
-        const float r2 = (xx * xx + xy * xy) + xz * xz;
-        s_j[cnt][tid] = j;
-        cnt += (valid && j != i && !(r2 > c.h2)) ? 1 : 0;
-    };
-    auto flush = [&]() {
-        for (int t = 0; t < cnt; t++) force_pair(f, p4, v4, pdi, s_j[t][tid], xs, vs, aux, c);
-        cnt = 0;
-    };
-    for (int dz = -1; dz <= 1; dz++) {
-        for (int dy = -1; dy <= 1; dy++) {
-            const RowRanges r = row_ranges(cx, cy, cz, dy, dz, c, b.cell_start);
-            uint32_t L = 0;
-            bool staged = false;
-            if (STAGED) staged = stage_row(st_mem[0], r, active, xs, L);
-            if (active) {
-                auto visit = [&](uint32_t jb, uint32_t je) {
-                    if (STAGED && staged)
-                        walk_range<true>(jb, je, st_mem[0].x, L, cnt, hit, flush);
-                    else
-                        walk_range<false>(jb, je, xs, 0u, cnt, hit, flush);
-                };
+
+    const uint32_t count = b.nbr_count[i];
+    if (count <= c.nbr_cap) {
+        const uint32_t* __restrict__ list = b.nbr + i;
+        if (count > 0) {
+            uint32_t j = list[0];
+            float4 q = load_f4(xr, j), vj = load_f4(vr, j);
+            float2 aj = load_f2(ar, j);
+            for (uint32_t t = 0; t < count; t++) {
+                const uint32_t jn = list[(size_t)min(t + 1u, count - 1u) * c.n];
+                const float4 qn = load_f4(xr, jn), vn = load_f4(vr, jn);
+                const float2 an = load_f2(ar, jn);
+                force_pair(f, p4, v4, pdi, q, vj, aj, c);
+                q = qn;
+                vj = vn;
+                aj = an;
+            }
+        }
+    } else {
+        // overflow: the 27 cells in reference order, pairs evaluated as found
+        const int32_t cx = floor_to_cell(p4.x / c.cell);
+        const int32_t cy = floor_to_cell(p4.y / c.cell);
+        const int32_t cz = floor_to_cell(p4.z / c.cell);
+        auto visit = [&](uint32_t jb, uint32_t je) {
+            for (uint32_t j = jb; j < je; j++) {
+                if (j == i) continue;
+                const float4 q = load_f4(xr, j);
+                const float xx = p4.x - q.x, xy = p4.y - q.y, xz = p4.z - q.z;
+                const float r2 = (xx * xx + xy * xy) + xz * xz;
+                if (r2 > c.h2) continue;
+                force_pair(f, p4, v4, pdi, q, load_f4(vr, j), load_f2(ar, j), c);
+            }
+        };
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++) {
+                const RowRanges r = row_ranges(cx, cy, cz, dy, dz, c, b.cell_start);
                 visit(r.b0, r.e0);
                 if (r.nr == 3) {
                     visit(r.b1, r.e1);
                     visit(r.b2, r.e2);
                 }
             }
-        }
     }
-    if (!active) return;
-    flush();
 
     const float pdb = di > 0.0f ? pi / c.rho0_sq : 0.0f;
     const float sb = (pdi + pdb) * c.rho0;
@@ -642,19 +706,13 @@ int launch_density(const StepBuffers& b, const SimConsts& c, const DevObject* ob
 
 int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, int block, int staged, hipStream_t s) {
     if (b.n == 0) return 0;
+    (void)staged;  // the force pass reads neighbour lists; row staging only affects the density pass
     const uint32_t bs = block == 128 ? 128u : 256u;
     const uint32_t nb = particle_blocks(b.n, bs);
-    if (bs == 128) {
-        if (staged)
-            hipLaunchKernelGGL((k_forces_integrate<128, true>), dim3(nb), dim3(128), 0, s, b, c, nb);
-        else
-            hipLaunchKernelGGL((k_forces_integrate<128, false>), dim3(nb), dim3(128), 0, s, b, c, nb);
-    } else {
-        if (staged)
-            hipLaunchKernelGGL((k_forces_integrate<256, true>), dim3(nb), dim3(256), 0, s, b, c, nb);
-        else
-            hipLaunchKernelGGL((k_forces_integrate<256, false>), dim3(nb), dim3(256), 0, s, b, c, nb);
-    }
+    if (bs == 128)
+        hipLaunchKernelGGL((k_forces_integrate<128>), dim3(nb), dim3(128), 0, s, b, c, nb);
+    else
+        hipLaunchKernelGGL((k_forces_integrate<256>), dim3(nb), dim3(256), 0, s, b, c, nb);
     return check_launch("forces_integrate");
 }
 
