@@ -11,10 +11,12 @@
 //   * pow(t,3) = (t*t)*t, pow(t,2) = t*t, pow(h,3) = (h*h)*h;
 //   * EOS power: integral exponent in [0,64] -> binary powering in double,
 //     rounded once to the working type; otherwise std::pow in double;
-//   * dot(a,b) = (ax*bx + ay*by) + az*bz; M*v sums columns left to right;
+//   * dot(a,b) = fma(az, bz, fma(ay, by, ax*bx)) (fused, as GPU shader
+//     compilers evaluate dot); M*v sums columns left to right, unfused;
 //   * a vector divided by a scalar (xij / |xij|, the viscous term over
 //     r^2 + 0.01h^2, the boundary normal, the SDF gradient) is multiplied by
-//     the IEEE reciprocal of the scalar; scalar quotients are IEEE divisions
+//     the IEEE reciprocal of the scalar, and so is r / h in the smoothing
+//     kernels (h is uniform); other scalar quotients are IEEE divisions
 //     (SPIR-V OpFDiv itself is only specified to 2.5 ulp under Vulkan);
 //   * the 1e-9 / 0.01 / 0.125 / 0.8 literals are taken as float literals;
 //   * float->int cell conversion saturates and maps NaN to 0;
@@ -129,7 +131,7 @@ inline double eos_pow<double>(double x, double e) {
 template <class R>
 struct Model {
     // derived uniforms
-    R g[3], h, h2, cell, rho0, B, gamma, nu, kwall, dt, mass;
+    R g[3], h, h2, inv_h, cell, rho0, B, gamma, nu, kwall, dt, mass;
     R sigma;       // cubic_spline_scale (model.cpp:231)
     R grad_sigma;  // grad_cubic_spline_scale / smooth_radius (kernels_3d.slang:80-81)
     R bb_lo[3], bb_hi[3];
@@ -146,7 +148,7 @@ struct Model {
 
     // kernels_3d.slang:62-76
     R W(R r) const {
-        const R q = r / h;
+        const R q = r * inv_h;
         if (q <= R(1)) {
             if (q > R(0.5)) {
                 const R t = R(1) - q;
@@ -160,7 +162,7 @@ struct Model {
     }
     // kernels_3d.slang:78-91 (note the extra 1/h in grad_sigma)
     R dW(R r) const {
-        const R q = r / h;
+        const R q = r * inv_h;
         if ((r > R(1e-9f)) && (q <= R(1))) {
             if (q > R(0.5)) {
                 const R t = R(1) - q;
@@ -187,6 +189,7 @@ void Model<float>::set_params(const ParamsC& p) {
     for (int a = 0; a < 3; a++) g[a] = p.gravity[a];
     h = p.smooth_radius;
     h2 = h * h;
+    inv_h = 1.0f / h;
     cell = p.cell_size > 0.0f ? p.cell_size : p.smooth_radius;
     rho0 = p.target_density;
     B = p.stiffness;
@@ -214,6 +217,7 @@ void Model<double>::set_params(const ParamsC& p) {
     for (int a = 0; a < 3; a++) g[a] = p.gravity[a];
     h = p.smooth_radius;
     h2 = h * h;
+    inv_h = 1.0 / h;
     cell = p.cell_size > 0.0f ? p.cell_size : p.smooth_radius;
     rho0 = p.target_density;
     B = p.stiffness;
@@ -234,7 +238,7 @@ void Model<double>::set_params(const ParamsC& p) {
 // ------------------------------------------------------------ grid sampling
 template <class R>
 static inline R dot3(const R* a, const R* b) {
-    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+    return std::fma(a[2], b[2], std::fma(a[1], b[1], a[0] * b[0]));
 }
 
 // SampleGrid, wcsph_with_boundary_model.slang:51-66 (base cell clamped to [0, n-2])

@@ -42,6 +42,7 @@ SimConsts make_consts(const vfs_params& p) {
     c.gz = p.gravity[2];
     c.h = p.smooth_radius;
     c.h2 = c.h * c.h;
+    c.inv_h = 1.0f / c.h;
     c.cell = p.cell_size > 0.0f ? p.cell_size : p.smooth_radius;
     c.rho0 = p.target_density;
     c.B = p.stiffness;
@@ -84,7 +85,6 @@ struct vfs_sim {
 
     float4 *x = nullptr, *v = nullptr, *xs = nullptr, *vs = nullptr, *bnd = nullptr, *accel = nullptr;
     float* rho = nullptr;
-    float2* aux = nullptr;
     uint32_t *nbr = nullptr, *nbr_count = nullptr;
     uint32_t *pid = nullptr, *pids = nullptr, *keys = nullptr, *sorted_keys = nullptr, *perm = nullptr,
              *offsets = nullptr, *cell_start = nullptr, *sort_ws = nullptr;
@@ -97,7 +97,7 @@ struct vfs_sim {
     bool stepped = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    vfs_tuning tuning{8, 16, 256, 0};
+    vfs_tuning tuning{8, 16, 128, 1};
 
     StepBuffers buffers() const {
         StepBuffers b{};
@@ -113,7 +113,6 @@ struct vfs_sim {
         b.vs = vs;
         b.pids = pids;
         b.rho = rho;
-        b.aux = aux;
         b.bnd = bnd;
         b.accel = accel;
         b.nbr = nbr;
@@ -254,7 +253,7 @@ void free_sim(vfs_sim* s) {
     if (!s) return;
     s->drop_graph();
     void* ptrs[] = {s->x,    s->v,    s->xs,   s->vs,          s->bnd,  s->accel,  s->rho,
-                    s->aux,  s->pid,  s->pids, s->keys,        s->sorted_keys, s->perm, s->offsets,
+                    s->pid,  s->pids, s->keys,        s->sorted_keys, s->perm, s->offsets,
                     s->cell_start, s->sort_ws, s->objs_dev, s->nbr, s->nbr_count};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -396,11 +395,12 @@ int vfs_sim_create(const vfs_params* params, const vfs_boundary_object* objects,
     int r = 0;
     r |= dev_alloc(&s->x, n);
     r |= dev_alloc(&s->v, n);
-    r |= dev_alloc(&s->xs, n);
+    // the density pass reads candidates in groups of kCandidateGroup and may
+    // read up to kCandidateGroup - 1 elements past the last particle
+    r |= dev_alloc(&s->xs, (size_t)n + kCandidateGroup);
     r |= dev_alloc(&s->vs, n);
     r |= dev_alloc(&s->bnd, n);
     r |= dev_alloc(&s->rho, n);
-    r |= dev_alloc(&s->aux, n);
     r |= dev_alloc(&s->keys, n);
     r |= dev_alloc(&s->sorted_keys, n);
     r |= dev_alloc(&s->perm, n);

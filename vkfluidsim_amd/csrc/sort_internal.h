@@ -72,18 +72,22 @@ struct StepBuffers {
     uint32_t* perm;
     uint32_t* offsets;     // n + 1: reference offsets table (n for empty keys), entry n = n
     uint32_t* cell_start;  // n + 1
-    float4* xs;
-    float4* vs;
+    float4* xs;          // n + kCandidateGroup entries; w: p / rho^2 once the density pass ran
+    float4* vs;          // w: mass / max(rho, 1e-6) once the density pass ran
     uint32_t* pids;      // nullable
     // per-particle results
     float* rho;
-    float2* aux;         // (p / rho^2, mass / max(rho, 1e-6))
     float4* bnd;         // (grad psi xyz, psi)
     float4* accel;       // nullable
-    uint32_t* nbr;       // nbr_cap * n: neighbour lists of the substep, slot-major
+    uint32_t* nbr;       // nbr_cap * n: neighbour lists of the substep in chunks of four,
+                         // chunk-major (uint4 chunk g of particle i at g * n + i)
     uint32_t* nbr_count; // n: list lengths (> nbr_cap: overflowed)
     uint32_t n;
 };
+
+// Candidates per group of the density pass's cell walk (its last group reads
+// past the range, so xs is padded by this many elements).
+constexpr uint32_t kCandidateGroup = 4;
 
 int launch_reorder_state(const StepBuffers& b, hipStream_t s);
 // block: 128 or 256 threads; staged: stage stencil rows of positions in LDS.

@@ -79,7 +79,7 @@ __device__ __forceinline__ int32_t floor_to_cell(float v) {
 // Everything the substep kernels read that is uniform over particles.
 struct SimConsts {
     float gx, gy, gz;
-    float h, h2, inv_unused;
+    float h, h2, inv_h;    // inv_h = 1 / h (IEEE)
     float cell;            // hash cell size
     float rho0, B, gamma;
     int32_t gamma_int;     // >= 0: integral EOS exponent, -1: general power
@@ -96,7 +96,8 @@ struct SimConsts {
     FastMod fm;            // % n
 };
 
-// Neighbour-list slots per particle (the lattice workloads have ~30 neighbours).
+// Neighbour-list slots per particle, a multiple of 4 (the lattice workloads
+// have ~30 neighbours).
 constexpr uint32_t kNeighbourCap = 64;
 
 // Boundary object as the kernels see it (column-major matrices).

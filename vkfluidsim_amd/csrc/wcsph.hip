@@ -7,9 +7,9 @@
 // assets/shaders/simulation/wcsph_with_boundary_model.slang and
 // kernels/kernels_3d.slang) operation by operation; the library is compiled
 // with -ffp-contract=off so that every float operation rounds once, as in the
-// CPU oracle; a vector divided by a scalar is multiplied by the IEEE
-// reciprocal of the scalar, scalar quotients are IEEE divisions (the oracle's
-// canonical reading of the shader arithmetic).  Neighbour enumeration visits the 27 hashed cells in the
+// CPU oracle; a vector divided by a scalar, and r / h in the smoothing kernels,
+// are multiplied by the IEEE reciprocal of the divisor, other scalar quotients
+// are IEEE divisions (the oracle's canonical reading of the shader arithmetic).  Neighbour enumeration visits the 27 hashed cells in the
 // reference order (z outer, y, x inner; spatial_hash_3d.slang:3-10) and each
 // bucket in ascending sorted index, so sums are accumulated in the reference
 // order.  When the three x-adjacent cells of a row hash to consecutive keys
@@ -20,9 +20,14 @@ namespace vfs {
 
 namespace {
 
+// dot(a, b) as fma(az, bz, fma(ay, by, ax * bx)).
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return __builtin_fmaf(az, bz, __builtin_fmaf(ay, by, ax * bx));
+}
+
 // CubicSpline, kernels_3d.slang:62-76
 __device__ __forceinline__ float kernel_w(float r, const SimConsts& c) {
-    const float q = r / c.h;
+    const float q = r * c.inv_h;
     if (q <= 1.0f) {
         if (q > 0.5f) {
             const float t = 1.0f - q;
@@ -37,7 +42,7 @@ __device__ __forceinline__ float kernel_w(float r, const SimConsts& c) {
 
 // GradCubicSpline, kernels_3d.slang:78-91 (grad_sigma carries the extra 1/h)
 __device__ __forceinline__ float kernel_dw(float r, const SimConsts& c) {
-    const float q = r / c.h;
+    const float q = r * c.inv_h;
     if ((r > 1e-9f) && (q <= 1.0f)) {
         if (q > 0.5f) {
             const float t = 1.0f - q;
@@ -158,7 +163,7 @@ __device__ void boundary_volume(const DevObject* __restrict__ objs, uint32_t n_o
             const float v0 = sample_grid(o.sdf, o.res, u);
             nrm[a] = (v1 - v0) * (1.0f / (2.0f * eps));
         }
-        const float modn = sqrtf((nrm[0] * nrm[0] + nrm[1] * nrm[1]) + nrm[2] * nrm[2]);
+        const float modn = sqrtf(dot3(nrm[0], nrm[1], nrm[2], nrm[0], nrm[1], nrm[2]));
         if (!(modn > 1e-9f)) continue;
         float rn[3];
         mat_vec(o.Rm, nrm, rn);
@@ -172,7 +177,7 @@ __device__ void boundary_volume(const DevObject* __restrict__ objs, uint32_t n_o
             const float xb = pos[a] - d * rn[a];
             xij[a] = pos[a] - xb;
         }
-        const float xm = sqrtf((xij[0] * xij[0] + xij[1] * xij[1]) + xij[2] * xij[2]);
+        const float xm = sqrtf(dot3(xij[0], xij[1], xij[2], xij[0], xij[1], xij[2]));
         psi += volume * kernel_w(xm, c);
         const float vg = volume * kernel_dw(xm, c);
         const float inv_xm = xm > 0.0f ? 1.0f / xm : 0.0f;
@@ -193,8 +198,9 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t nblocks) {
 }
 
 // Raw buffer resources over the particle arrays: 32-bit offsets and hardware
-// range checks (a load past num_records bytes returns zeros), so candidate
-// groups need neither 64-bit address arithmetic nor index clamping.
+// range checks (a load past num_records bytes returns zeros), so loads by
+// neighbour index need no 64-bit address arithmetic, and the prefetch of a
+// stale queue slot cannot fault.
 constexpr int kBufferDword3 = 0x00020000;  // raw (untyped) buffer access on gfx9
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
@@ -215,15 +221,15 @@ __device__ __forceinline__ float2 load_f2(__amdgpu_buffer_rsrc_t r, uint32_t ind
     return v;
 }
 
-// In-range neighbours are appended to a per-lane LDS queue and evaluated once
-// any active lane holds kQueue of them, so the pair arithmetic runs with most
-// lanes busy instead of with the ~15% of lanes whose candidate happens to be in
-// range.  Queued pairs are evaluated in arrival order: sums keep the reference
-// order.  Candidates are read kUnroll at a time (all loads issued before the
-// tests); every candidate writes the next queue slot and only advances the
-// count when it is in range, so the test has no branch.  A queue can therefore
-// hold up to kQueue - 1 + kUnroll entries before the flush check.
-constexpr int kUnroll = 8;
+// The cell walk appends in-range neighbours to a per-lane LDS queue and
+// moves them into the neighbour list once any active lane holds kQueue of
+// them, so the list stores run with most lanes busy.  Queued neighbours keep
+// their arrival order (the reference order).  Candidates are read kUnroll at
+// a time (all loads issued before the tests); every candidate writes the next
+// queue slot and only advances the count when it is in range, so the test has
+// no branch.  A queue can therefore hold up to kQueue - 1 + kUnroll entries
+// before the flush check.
+constexpr int kUnroll = (int)kCandidateGroup;
 constexpr int kQueue = 16;
 constexpr int kQueueCap = kQueue + kUnroll;
 
@@ -258,68 +264,60 @@ __device__ __forceinline__ RowRanges row_ranges(int32_t cx, int32_t cy, int32_t 
     return r;
 }
 
-// Optional LDS staging of a block's stencil row: the particles of a block sit in
-// a few consecutive cells, so their row ranges overlap in one short span of
-// sorted indices that the block loads once.  Rows whose span exceeds the stage
-// read global memory.
-constexpr uint32_t kStage = 640;
+// Optional wave-level LDS staging of a stencil row.  The 64 particles of a
+// wave lie in a few consecutive cells of one x-row of cells, so their ranges
+// for row (dy, dz) fall in one short span of sorted indices: the wave copies
+// the span to its LDS slice with coalesced loads and the lanes read their
+// candidates from there.  Rows whose span exceeds the slice read global
+// memory.  No block barrier: the copy and the reads are ordered within the
+// wave (LDS executes one wave's instructions in issue order).
+constexpr uint32_t kWaveStage = 192;  // three coalesced loads per lane
 
-template <int WAVES>
-struct RowStage {
-    float4 x[kStage];
-    uint32_t lo[WAVES], hi[WAVES];
-};
-
-// Block-wide span [L, H) of the lanes' row ranges, staged when it fits.  Two
-// barriers per row: the span slots of row r+1 are written only after every wave
-// passed the second barrier of row r (its slot reads are done), and the stage
-// of row r+1 only after the first barrier of row r+1 (every wave finished
-// reading the stage of row r).
-template <int WAVES>
-__device__ __forceinline__ bool stage_row(RowStage<WAVES>& st, const RowRanges& r, bool active,
+// Stages row (dy, dz) for the wave when the lanes' ranges fit one slice: the
+// span runs from the first lane's lowest begin to the last lane's highest end
+// (sorted keys ascend over the lanes, so this is the common case) and a wave
+// vote confirms that every lane's ranges lie inside.  Returns the wave-uniform
+// decision; L is the sorted index of slice entry 0.
+__device__ __forceinline__ bool stage_row(float4* __restrict__ st, const RowRanges& r, bool active,
                                           const float4* __restrict__ xs, uint32_t& L) {
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-    if (active) {
-        auto span = [&](uint32_t jb, uint32_t je) {
-            if (je > jb) {
-                lo = min(lo, jb);
-                hi = max(hi, je);
-            }
-        };
-        span(r.b0, r.e0);
-        if (r.nr == 3) {
-            span(r.b1, r.e1);
-            span(r.b2, r.e2);
+    auto span = [&](uint32_t jb, uint32_t je) {
+        if (je > jb) {
+            lo = min(lo, jb);
+            hi = max(hi, je);
         }
+    };
+    span(r.b0, r.e0);
+    if (r.nr == 3) {
+        span(r.b1, r.e1);
+        span(r.b2, r.e2);
     }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        lo = min(lo, (uint32_t)__shfl_xor(lo, o, 64));
-        hi = max(hi, (uint32_t)__shfl_xor(hi, o, 64));
+    const uint32_t wl = (uint32_t)__shfl(lo, 0, 64);
+    const uint32_t wh = (uint32_t)__shfl(hi, 63, 64);
+    const bool fits = !active || (lo >= wl && hi <= wh);
+    const bool staged = __all(fits) && wh > wl && wh - wl <= kWaveStage;
+    if (staged) {
+        // every lane's reads of the previous row precede these writes
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t lane = threadIdx.x & 63u, n = wh - wl;
+        float4 a0, a1, a2;
+        if (lane < n) a0 = xs[wl + lane];
+        if (lane + 64u < n) a1 = xs[wl + lane + 64u];
+        if (lane + 128u < n) a2 = xs[wl + lane + 128u];
+        if (lane < n) st[lane] = a0;
+        if (lane + 64u < n) st[lane + 64u] = a1;
+        if (lane + 128u < n) st[lane + 128u] = a2;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        st.lo[w] = lo;
-        st.hi[w] = hi;
-    }
-    __syncthreads();
-    uint32_t bl = st.lo[0], bh = st.hi[0];
-#pragma unroll
-    for (int k = 1; k < WAVES; k++) {
-        bl = min(bl, st.lo[k]);
-        bh = max(bh, st.hi[k]);
-    }
-    const bool staged = bh > bl && bh - bl <= kStage;
-    if (staged)
-        for (uint32_t t = threadIdx.x; t < bh - bl; t += 64u * WAVES) st.x[t] = xs[bl + t];
-    __syncthreads();
-    L = bl;
+    L = wl;
     return staged;
 }
 
 // Visits candidates jb..je-1 in order with kUnroll loads in flight per lane:
 // whole groups first (every candidate valid), then one masked tail group whose
-// extra loads read neighbouring particles (or zeros past the array).
+// extra loads read neighbouring particles (or the padding past the array).
 // hit(j, q, valid) appends j to the queue when valid and in range.
 template <class Load, class Hit, class Flush>
 __device__ __forceinline__ void walk_range(uint32_t jb, uint32_t je, Load&& load, const int& cnt, Hit&& hit,
@@ -328,7 +326,7 @@ __device__ __forceinline__ void walk_range(uint32_t jb, uint32_t je, Load&& load
     for (; j0 + (uint32_t)kUnroll <= je; j0 += (uint32_t)kUnroll) {
         float4 q[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) q[u] = load(j0 + (uint32_t)u);
+        for (int u = 0; u < kUnroll; u++) q[u] = load(j0, u);
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) hit(j0 + (uint32_t)u, q[u], true);
         if (__any(cnt >= kQueue)) flush();
@@ -336,7 +334,7 @@ __device__ __forceinline__ void walk_range(uint32_t jb, uint32_t je, Load&& load
     if (j0 < je) {
         float4 q[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) q[u] = load(j0 + (uint32_t)u);
+        for (int u = 0; u < kUnroll; u++) q[u] = load(j0, u);
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) hit(j0 + (uint32_t)u, q[u], j0 + (uint32_t)u < je);
     }
@@ -358,15 +356,19 @@ __global__ __launch_bounds__(256) void k_reorder_state(StepBuffers b, uint32_t n
 
 // Boundary volume, CalculateDensities (slang:153-197), the per-particle EOS
 // terms the force pass reads for every neighbour, and this substep's neighbour
-// list: the in-range neighbours j != i in reference order, stored slot-major
-// (nbr[t * n + i]) so that the force pass reads it coalesced.  A particle with
-// more than nbr_cap neighbours stores its full count and the force pass walks
-// its cells again.
+// list.  The cell walk only finds the in-range neighbours j (self included,
+// once per visit of its bucket) in reference order and stores them in chunks
+// of four, chunk-major (chunk g of particle i at g * n + i), so that list
+// reads and writes are coalesced and each store fills 16 bytes.  The density
+// sum then walks the particle's own list, every lane busy until its own count.
+// A particle with more than nbr_cap neighbours stores its count only; its
+// density, and its forces in the force pass, walk the cells again.
 template <int BLOCK, bool STAGED>
 __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, const DevObject* __restrict__ objs,
                                                    uint32_t n_obj, uint32_t nblocks) {
     __shared__ uint32_t s_j[kQueueCap][BLOCK];
-    __shared__ RowStage<STAGED ? BLOCK / 64 : 1> st_mem[1];
+    // wave slices of staged rows, padded for the masked tail group's reads
+    __shared__ float4 s_stage[STAGED ? BLOCK / 64 : 1][kWaveStage + kUnroll];
     const int tid = threadIdx.x;
     const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + tid;
     const bool active = i < c.n;
@@ -380,84 +382,119 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, c
     const int32_t cz = floor_to_cell(pos[2] / c.cell);
     const float4* __restrict__ xs = b.xs;
     const auto xr = make_rsrc(xs, c.n * 16u);
-    uint32_t* __restrict__ list = b.nbr;
-    float density = 0.0f;
+    uint4* __restrict__ list = reinterpret_cast<uint4*>(b.nbr);
+    uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // the list chunk being filled
     int cnt = 0;
     uint32_t listed = 0;
     auto hit = [&](uint32_t j, const float4& q, bool valid) {
         const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
-        const float r2 = (ox * ox + oy * oy) + oz * oz;
+        const float r2 = dot3(ox, oy, oz, ox, oy, oz);
         s_j[cnt][tid] = j;
         cnt += (valid && !(r2 > c.h2)) ? 1 : 0;
     };
-    // Queued neighbours in order, the next one's position in flight.  The slot
-    // after the last queued one may hold any index; the buffer range check
-    // makes its load harmless.
+    // queued neighbours in order into the list chunks
     auto flush = [&]() {
-        if (cnt > 0) {
-            uint32_t j = s_j[0][tid];
-            float4 q = load_f4(xr, j);
-            for (int t = 0; t < cnt; t++) {
-                const uint32_t jn = s_j[min(t + 1, kQueueCap - 1)][tid];
-                const float4 qn = load_f4(xr, jn);
-                const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
-                const float r2 = (ox * ox + oy * oy) + oz * oz;
-                density += c.mass * kernel_w(sqrtf(r2), c);
-                if (j != i) {
-                    if (listed < c.nbr_cap) list[(size_t)listed * c.n + i] = j;
-                    listed++;
-                }
-                j = jn;
-                q = qn;
-            }
+        for (int t = 0; t < cnt; t++) {
+            const uint32_t j = s_j[t][tid];
+            const uint32_t k = listed & 3u;
+            e0 = k == 0u ? j : e0;
+            e1 = k == 1u ? j : e1;
+            e2 = k == 2u ? j : e2;
+            e3 = k == 3u ? j : e3;
+            if (k == 3u && listed < c.nbr_cap) list[(size_t)(listed >> 2) * c.n + i] = make_uint4(e0, e1, e2, e3);
+            listed++;
         }
         cnt = 0;
     };
-    for (int dz = -1; dz <= 1; dz++) {
-        for (int dy = -1; dy <= 1; dy++) {
-            const RowRanges r = row_ranges(cx, cy, cz, dy, dz, c, b.cell_start);
-            uint32_t L = 0;
-            bool staged = false;
-            if (STAGED) staged = stage_row(st_mem[0], r, active, xs, L);
-            if (active) {
-                auto visit = [&](uint32_t jb, uint32_t je) {
-                    if (STAGED && staged)
-                        walk_range(
-                            jb, je, [&](uint32_t j) { return st_mem[0].x[min(j - L, kStage - 1u)]; }, cnt, hit,
-                            flush);
-                    else
-                        walk_range(jb, je, [&](uint32_t j) { return load_f4(xr, j); }, cnt, hit, flush);
-                };
-                visit(r.b0, r.e0);
-                if (r.nr == 3) {
-                    visit(r.b1, r.e1);
-                    visit(r.b2, r.e2);
-                }
+    auto cells = [&](auto&& visit) {
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++) {
+                const RowRanges r = row_ranges(cx, cy, cz, dy, dz, c, b.cell_start);
+                visit(r);
+            }
+    };
+    cells([&](const RowRanges& r) {
+        uint32_t L = 0;
+        bool staged = false;
+        if (STAGED) staged = stage_row(s_stage[STAGED ? tid >> 6 : 0], r, active, xs, L);
+        if (active) {
+            auto visit = [&](uint32_t jb, uint32_t je) {
+                if (STAGED && staged)
+                    walk_range(
+                        jb, je, [&](uint32_t j0, int u) { return s_stage[STAGED ? tid >> 6 : 0][j0 - L + u]; },
+                        cnt, hit, flush);
+                else
+                    walk_range(jb, je, [&](uint32_t j0, int u) { return (xs + j0)[u]; }, cnt, hit, flush);
+            };
+            visit(r.b0, r.e0);
+            if (r.nr == 3) {
+                visit(r.b1, r.e1);
+                visit(r.b2, r.e2);
             }
         }
-    }
+    });
     if (!active) return;
     flush();
+    if ((listed & 3u) != 0u && listed <= c.nbr_cap) list[(size_t)(listed >> 2) * c.n + i] = make_uint4(e0, e1, e2, e3);
+    b.nbr_count[i] = listed;
+
+    float density = 0.0f;
+    auto add = [&](const float4& q) {
+        const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
+        density += c.mass * kernel_w(sqrtf(dot3(ox, oy, oz, ox, oy, oz)), c);
+    };
+    if (listed <= c.nbr_cap) {
+        // the lane's own list, just written (same-address accesses of a wave
+        // are ordered); entries past listed in the last chunk are ignored
+        const uint4* __restrict__ own = list + i;
+        const uint32_t groups = (listed + 3u) >> 2;
+        for (uint32_t gi = 0; gi < groups; gi++) {
+            const uint4 e = own[(size_t)gi * c.n];
+            const float4 q0 = load_f4(xr, e.x), q1 = load_f4(xr, e.y), q2 = load_f4(xr, e.z),
+                         q3 = load_f4(xr, e.w);
+            const uint32_t t = 4u * gi;
+            add(q0);
+            if (t + 1u < listed) add(q1);
+            if (t + 2u < listed) add(q2);
+            if (t + 3u < listed) add(q3);
+        }
+    } else {
+        cells([&](const RowRanges& r) {
+            auto visit = [&](uint32_t jb, uint32_t je) {
+                for (uint32_t j = jb; j < je; j++) {
+                    const float4 q = xs[j];
+                    const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
+                    if (!(dot3(ox, oy, oz, ox, oy, oz) > c.h2)) add(q);
+                }
+            };
+            visit(r.b0, r.e0);
+            if (r.nr == 3) {
+                visit(r.b1, r.e1);
+                visit(r.b2, r.e2);
+            }
+        });
+    }
     density += c.rho0 * psi;
     b.rho[i] = density;
     const float p = eos(density, c);
     const float pd = density > 0.0f ? p / (density * density) : 0.0f;
     const float mrho = c.mass / fmaxf(density, 1e-6f);
-    b.aux[i] = make_float2(pd, mrho);
+    // the force pass reads a neighbour's EOS terms with its position and velocity
+    reinterpret_cast<float*>(b.xs)[4 * (size_t)i + 3] = pd;
+    reinterpret_cast<float*>(b.vs)[4 * (size_t)i + 3] = mrho;
     b.bnd[i] = make_float4(g[0], g[1], g[2], psi);
-    b.nbr_count[i] = listed;
 }
 
 struct ForceAcc {
     float pfx, pfy, pfz, vfx, vfy, vfz;
 };
 
-// Pressure (slang:205-262) and viscosity (slang:273-324) terms of one pair
-// (neighbour position q, velocity vj, EOS terms aj).
+// Pressure (slang:205-262) and viscosity (slang:273-324) terms of one pair:
+// neighbour position and p/rho^2 in q, velocity and mass/max(rho, 1e-6) in vj.
 __device__ __forceinline__ void force_pair(ForceAcc& f, const float4& p4, const float4& v4, float pdi, const float4& q,
-                                           const float4& vj, const float2& aj, const SimConsts& c) {
+                                           const float4& vj, const SimConsts& c) {
     const float xx = p4.x - q.x, xy = p4.y - q.y, xz = p4.z - q.z;
-    const float r2 = (xx * xx + xy * xy) + xz * xz;
+    const float r2 = dot3(xx, xy, xz, xx, xy, xz);
     const float r = sqrtf(r2);
     float nx, ny, nz;
     if (r > 0.0f) {
@@ -471,12 +508,12 @@ __device__ __forceinline__ void force_pair(ForceAcc& f, const float4& p4, const 
         nz = 0.0f;
     }
     const float G = kernel_dw(r, c);
-    const float s = (c.mass * (pdi + aj.x)) * G;
+    const float s = (c.mass * (pdi + q.w)) * G;
     f.pfx += s * nx;
     f.pfy += s * ny;
     f.pfz += s * nz;
     const float ux = v4.x - vj.x, uy = v4.y - vj.y, uz = v4.z - vj.z;
-    const float sv = aj.y * ((ux * xx + uy * xy) + uz * xz);
+    const float sv = vj.w * dot3(ux, uy, uz, xx, xy, xz);
     const float inv_den = 1.0f / (r2 + c.visc_eps_h2);
     f.vfx += (sv * (nx * G)) * inv_den;
     f.vfy += (sv * (ny * G)) * inv_den;
@@ -494,29 +531,32 @@ __global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimCo
     const float4 p4 = b.xs[i];
     const float4 v4 = b.vs[i];
     const float di = b.rho[i];
-    const float pdi = b.aux[i].x;
+    const float pdi = p4.w;
     const float pi = eos(di, c);
     const auto xr = make_rsrc(b.xs, c.n * 16u);
     const auto vr = make_rsrc(b.vs, c.n * 16u);
-    const auto ar = make_rsrc(b.aux, c.n * 8u);
     ForceAcc f{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 
     const uint32_t count = b.nbr_count[i];
     if (count <= c.nbr_cap) {
-        const uint32_t* __restrict__ list = b.nbr + i;
-        if (count > 0) {
-            uint32_t j = list[0];
-            float4 q = load_f4(xr, j), vj = load_f4(vr, j);
-            float2 aj = load_f2(ar, j);
-            for (uint32_t t = 0; t < count; t++) {
-                const uint32_t jn = list[(size_t)min(t + 1u, count - 1u) * c.n];
-                const float4 qn = load_f4(xr, jn), vn = load_f4(vr, jn);
-                const float2 an = load_f2(ar, jn);
-                force_pair(f, p4, v4, pdi, q, vj, aj, c);
-                q = qn;
-                vj = vn;
-                aj = an;
-            }
+        // chunks of four neighbours; entries past count in the last chunk are
+        // earlier neighbours or 0, loaded and ignored; self entries are skipped
+        // as in the reference loops
+        const uint4* __restrict__ list = reinterpret_cast<const uint4*>(b.nbr) + i;
+        const uint32_t groups = (count + 3u) >> 2;
+        uint4 e = groups ? list[0] : make_uint4(0u, 0u, 0u, 0u);
+        for (uint32_t g = 0; g < groups; g++) {
+            const uint4 en = list[(size_t)min(g + 1u, groups - 1u) * c.n];
+            const float4 q0 = load_f4(xr, e.x), w0 = load_f4(vr, e.x);
+            const float4 q1 = load_f4(xr, e.y), w1 = load_f4(vr, e.y);
+            const float4 q2 = load_f4(xr, e.z), w2 = load_f4(vr, e.z);
+            const float4 q3 = load_f4(xr, e.w), w3 = load_f4(vr, e.w);
+            const uint32_t t = 4u * g;
+            if (e.x != i) force_pair(f, p4, v4, pdi, q0, w0, c);
+            if (t + 1u < count && e.y != i) force_pair(f, p4, v4, pdi, q1, w1, c);
+            if (t + 2u < count && e.z != i) force_pair(f, p4, v4, pdi, q2, w2, c);
+            if (t + 3u < count && e.w != i) force_pair(f, p4, v4, pdi, q3, w3, c);
+            e = en;
         }
     } else {
         // overflow: the 27 cells in reference order, pairs evaluated as found
@@ -528,9 +568,9 @@ __global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimCo
                 if (j == i) continue;
                 const float4 q = load_f4(xr, j);
                 const float xx = p4.x - q.x, xy = p4.y - q.y, xz = p4.z - q.z;
-                const float r2 = (xx * xx + xy * xy) + xz * xz;
+                const float r2 = dot3(xx, xy, xz, xx, xy, xz);
                 if (r2 > c.h2) continue;
-                force_pair(f, p4, v4, pdi, q, load_f4(vr, j), load_f2(ar, j), c);
+                force_pair(f, p4, v4, pdi, q, load_f4(vr, j), c);
             }
         };
         for (int dz = -1; dz <= 1; dz++)
