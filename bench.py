@@ -8,13 +8,16 @@ viscosity + integrate, over every particle.  Inputs are resident in HBM
 before the timed region; the timed region replays the captured substep graph.
 
 Prints one JSON line: particle-substeps/s, the roofline of the dominant kernel
-(algorithmic bytes per launch / mean HIP-event duration on the simulation
-stream), and the CPU oracle timed on a bounded sample of the same workload.
+(compulsory bytes per launch / mean HIP-event duration on the simulation
+stream), its measured HBM traffic per launch (two short child runs of this
+script under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy of known
+size), and the CPU oracle timed on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 from pathlib import Path
@@ -26,16 +29,113 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "particle-steps/s @1M WCSPH; batched vs looped scenes/s; adjoint/forward cost"
 HBM_PEAK = 8.0e12
-# Algorithmic (compulsory) bytes per particle of each stage: every array read or
-# written once, neighbour re-reads as cache hits, grids L2-resident.
-B_STAGE = {
-    "sort": 28,              # keys read twice, sorted keys + permutation written, table traffic
-    "cell_ranges": 8,        # sorted key read, range entry written
-    "reorder": 52,           # permutation 4 + gather x, v 24 + write x, v 24
-    "density": 16,           # x 12 read, rho 4 written (boundary volume fused)
-    "forces_integrate": 68,  # x, v, rho 28 read + x, v 24 written; + hash of new x (key 4 written, x 12)
-}
-B_ALG = 172  # per particle-substep, whole substep
+# Whole-substep compulsory bytes per particle (hash 16, sort 28, ranges 8,
+# reorder 52, density 16, forces + integrate 52): every array read or written
+# once, neighbour re-reads as cache hits, grids L2-resident.
+B_ALG = 172
+
+
+def stage_bytes(list_mean: float) -> dict:
+    """Compulsory bytes per particle of each stage of this engine, with the
+    neighbour list (4 bytes per entry, list_mean entries per particle) that the
+    density pass writes and the force pass reads."""
+    L = 4.0 * list_mean
+    return {
+        "sort": 28.0,              # keys in, sorted keys + permutation out, table fill
+        "cell_ranges": 8.0,        # offsets table in, cell starts out
+        "reorder": 52.0,           # permutation 4, gather x v 32, write x v 32 (float4) less the w lanes
+        # positions 16 in; rho 4, boundary (grad, psi) 16, list count 4, EOS terms 8 out; list out
+        "density": 48.0 + L,
+        # x v rho boundary count 56 in + list; x v key 36 out
+        "forces_integrate": 92.0 + L,
+    }
+
+
+# kernel-name prefixes of the stages in the rocprofv3 databases
+STAGE_KERNELS = {"density": "k_density", "forces_integrate": "k_forces_integrate", "reorder": "k_reorder_state",
+                 "cell_ranges": "k_cell_start_scan", "sort": "k_radix_pass"}
+CALIBRATION_MIB = 256
+
+
+def kernel_base(name: str) -> str:
+    """'void vfs::(anonymous namespace)::k_density<128, true>(...)' -> 'k_density'."""
+    name = name.replace("(anonymous namespace)::", "")
+    return name.split("(")[0].split("<")[0].split("::")[-1].split(" ")[-1]
+
+
+def measure_traffic(workload: str, timeout: float = 240.0) -> dict:
+    """HBM bytes per launch of each stage kernel: two child runs of this script
+    under rocprofv3 --pmc (FETCH_SIZE, then WRITE_SIZE), each counter scaled by
+    what it reads for a device-to-device copy of CALIBRATION_MIB MiB (which
+    moves that many bytes each way).  Returns {} when rocprofv3 is unavailable
+    or fails; runs before this process touches the GPU."""
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not Path(prof).exists():
+        return {"error": "rocprofv3 not found"}
+    out = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        per_counter = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = Path(tmp) / counter
+            cmd = [prof, "--pmc", counter, "-d", str(d), "-o", "run", "--", sys.executable, str(ROOT / "bench.py"),
+                   "--traffic-probe", "--workload", workload]
+            try:
+                r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout)
+            except subprocess.TimeoutExpired:
+                return {"error": f"rocprofv3 --pmc {counter} timed out"}
+            dbs = list(d.rglob("*results.db"))
+            if r.returncode != 0 and not dbs:
+                return {"error": f"rocprofv3 --pmc {counter} failed: {r.stderr[-300:]}"}
+            if not dbs:
+                return {"error": f"rocprofv3 --pmc {counter}: no database"}
+            rows = sqlite3.connect(str(dbs[0])).execute(
+                "select kernel_name, value, grid_size from counters_collection where counter_name = ?",
+                (counter,)).fetchall()
+            per_counter[counter] = rows
+        calib = {}
+        for counter, rows in per_counter.items():
+            copies = [(g, v) for name, v, g in rows if "copyBuffer" in name]
+            if not copies:
+                return {"error": "calibration copy not found"}
+            calib[counter] = (CALIBRATION_MIB << 20) / (max(copies)[1] * 1024.0)
+        for stage, prefix in STAGE_KERNELS.items():
+            tot = 0.0
+            for counter, rows in per_counter.items():
+                vals = [v for name, v, g in rows if kernel_base(name) == prefix]
+                if not vals:
+                    tot = None
+                    break
+                tot += sum(vals) / len(vals) * 1024.0 * calib[counter]
+            if tot is not None:
+                out[stage] = tot
+        out["calibration"] = {k: round(v, 4) for k, v in calib.items()}
+    return out
+
+
+def traffic_probe(workload: str) -> int:
+    """Child of measure_traffic: one calibration copy, then two substeps."""
+    import vkfluidsim_amd as V
+    from vkfluidsim_amd import configs as CF
+
+    nbytes = CALIBRATION_MIB << 20
+    src, dst = V.DeviceArray(nbytes), V.DeviceArray(nbytes)
+    V._lib.call("vfs_memcpy_d2d", dst.ptr, src.ptr, nbytes)
+    src.free()
+    dst.free()
+    w = CF.WORKLOADS[workload]()
+    grids = [V.volume_map(kind, sp, res, w.h) for kind, sp, res in w.shapes]
+    sim = V.Sim(V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos), grids)
+    sim.set_state(w.x, w.v)
+    sim.substeps(3)
+    sim.synchronize()
+    sim.close()
+    return 0
 
 
 def cpu_baseline(w, grids, steps: int) -> dict:
@@ -58,6 +158,50 @@ def cpu_baseline(w, grids, steps: int) -> dict:
                       f"(FP32 oracle, OpenMP, {dt:.1f} s)"}
 
 
+def summarize_stats(db: str) -> str:
+    """Text table of a rocprofv3 --kernel-trace --stats database (top_kernels)."""
+    import sqlite3
+
+    rows = sqlite3.connect(db).execute(
+        "select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
+    lines = [f"{'kernel':34s} {'calls':>6s} {'total_us':>11s} {'avg_us':>9s} {'pct':>6s}"]
+    for name, calls, total, avg, pct in rows:
+        short = name.replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0].replace("vfs::", "")
+        lines.append(f"{short:34s} {calls:6d} {total / 1e3:11.1f} {avg / 1e3:9.3f} {pct:6.2f}")
+    return "\n".join(lines)
+
+
+def summarize_pmc(fetch_db: str, write_db: str) -> str:
+    """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of two rocprofv3
+    --pmc databases of the traffic probe, and the HBM bytes per launch after
+    scaling by the calibration copy."""
+    import sqlite3
+
+    per = {}
+    for counter, db in (("FETCH_SIZE", fetch_db), ("WRITE_SIZE", write_db)):
+        per[counter] = sqlite3.connect(db).execute(
+            "select kernel_name, value, grid_size from counters_collection where counter_name = ?",
+            (counter,)).fetchall()
+    calib = {}
+    for counter, rows in per.items():
+        copies = [(g, v) for name, v, g in rows if "copyBuffer" in name]
+        calib[counter] = (CALIBRATION_MIB << 20) / (max(copies)[1] * 1024.0)
+    names = sorted({kernel_base(n) for rows in per.values() for n, _, _ in rows})
+    lines = [f"calibration: {CALIBRATION_MIB} MiB device-to-device copy reads FETCH_SIZE x "
+             f"{calib['FETCH_SIZE']:.4f}, writes WRITE_SIZE x {calib['WRITE_SIZE']:.4f}",
+             f"{'kernel':30s} {'n':>3s} {'FETCH_KiB':>11s} {'WRITE_KiB':>11s} {'HBM_MB/launch':>14s}"]
+    for k in names:
+        if "copyBuffer" in k or "fillBuffer" in k:
+            continue
+        f = [v for n, v, _ in per["FETCH_SIZE"] if kernel_base(n) == k]
+        w = [v for n, v, _ in per["WRITE_SIZE"] if kernel_base(n) == k]
+        fa = sum(f) / len(f) if f else 0.0
+        wa = sum(w) / len(w) if w else 0.0
+        mb = (fa * calib["FETCH_SIZE"] + wa * calib["WRITE_SIZE"]) * 1024.0 / 1e6
+        lines.append(f"{k:30s} {len(f):3d} {fa:11.1f} {wa:11.1f} {mb:14.1f}")
+    return "\n".join(lines)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,15 +211,28 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
-                    help="first run one device-to-device copy of this size (counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc child runs")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--summarize-stats", metavar="DB", help="print a rocprofv3 --stats database as text")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
+                    help="print the HBM traffic of two --pmc databases of --traffic-probe runs")
     args = ap.parse_args()
+    if args.summarize_stats:
+        print(summarize_stats(args.summarize_stats))
+        return 0
+    if args.summarize_pmc:
+        print(summarize_pmc(*args.summarize_pmc))
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the substep engine is a "
               "single-GPU kernel path; multi-GPU sharding is a design note, not a measured path.",
               file=sys.stderr)
         return 2
+    if args.traffic_probe:
+        return traffic_probe(args.workload)
+
+    traffic = {} if args.no_traffic else measure_traffic(args.workload)
 
     import vkfluidsim_amd as V
     from vkfluidsim_amd import configs as CF
@@ -83,12 +240,6 @@ def main() -> int:
     if V.device_count() < 1:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
-    if args.calibrate_copy_mib > 0:
-        nbytes = args.calibrate_copy_mib << 20
-        src, dst = V.DeviceArray(nbytes), V.DeviceArray(nbytes)
-        V._lib.call("vfs_memcpy_d2d", dst.ptr, src.ptr, nbytes)
-        src.free()
-        dst.free()
     w = CF.WORKLOADS[args.workload]()
     grids = [V.volume_map(kind, sp, res, w.h) for kind, sp, res in w.shapes]
     params = V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos)
@@ -104,6 +255,7 @@ def main() -> int:
     ms_per_step = 1e3 * elapsed / args.steps
     value = w.n * args.steps / elapsed
 
+    list_mean, list_max, list_over = sim.neighbour_stats()
     x, v, rho, _ = sim.get_state()
     finite = bool(np.isfinite(x).all() and np.isfinite(v).all() and np.isfinite(rho).all())
     lo = np.asarray(w.box_pos, np.float32)
@@ -111,7 +263,8 @@ def main() -> int:
     end_state = {"substeps_run": args.warmup + args.steps, "finite": finite,
                  "max_speed": float(np.linalg.norm(v, axis=1).max()) if w.n else 0.0,
                  "outside_box": int(np.any((x < lo) | (x > hi), axis=1).sum()),
-                 "rho_mean": float(rho.mean()) if w.n else 0.0}
+                 "rho_mean": float(rho.mean()) if w.n else 0.0,
+                 "neighbour_list": {"mean": round(list_mean, 3), "max": list_max, "overflowed": list_over}}
     stages = sim.profile(args.profile_steps)
     sim.close()
 
@@ -122,11 +275,11 @@ def main() -> int:
     per["sort"] = sort_ms
     dom = max(per, key=per.get)
     dom_launch_ms = per[dom] / max(1, launches.get(dom, 1)) if dom != "sort" else sort_ms
-    achieved = B_STAGE[dom] * w.n / (dom_launch_ms * 1e-3)
-    mem_class = [k for k in ("sort", "cell_ranges", "reorder")]
+    sb = stage_bytes(list_mean)
+    achieved = sb[dom] * w.n / (dom_launch_ms * 1e-3)
+    mem_class = ["sort", "cell_ranges", "reorder"]
     mem_ms = sum(per[k] for k in mem_class)
-    mem_bytes = sum(B_STAGE[k] for k in mem_class) * w.n
-    step_ms_events = sum(stage_ms.values())
+    mem_bytes = sum(sb[k] for k in mem_class) * w.n
 
     out = {
         "metric": METRIC,
@@ -144,13 +297,16 @@ def main() -> int:
         "config": {"workload": w.name, "n_particles": w.n, "smooth_radius": w.h, "dt": w.dt,
                    "boundary_grids": [list(g.resolution) for g in grids], "substep_graph": True},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "bytes_per_particle": B_STAGE[dom]},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": traffic.get(dom) if dom in traffic else None,
+                     "algorithmic_bytes": sb[dom] * w.n, "bytes_per_particle": round(sb[dom], 2),
+                     "launch_ms": dom_launch_ms},
         "whole_step_hbm_frac": B_ALG * value / HBM_PEAK,
         "memory_class_hbm_frac": (mem_bytes / (mem_ms * 1e-3)) / HBM_PEAK if mem_ms > 0 else None,
+        "traffic_per_launch": {k: round(v) for k, v in traffic.items() if k in STAGE_KERNELS} or traffic or None,
         "stages_ms": {k: round(v, 5) for k, v in stage_ms.items()},
         "stage_launches": launches,
-        "event_sum_ms_per_step": step_ms_events,
+        "event_sum_ms_per_step": sum(stage_ms.values()),
         "end_state": end_state,
     }
     if not args.no_cpu_baseline:

@@ -37,6 +37,10 @@ typedef enum vfs_status {
 #define VFS_FLAG_NO_GRAPH 0x2u    /* launch substeps eagerly instead of replaying a hipGraph */
 #define VFS_FLAG_STORE_ACCEL 0x4u /* keep the per-particle acceleration buffer up to date */
 
+/* Neighbour-list entries kept per particle and substep (a list longer than
+ * this falls back to walking the cells, with identical results). */
+#define VFS_NEIGHBOUR_CAPACITY 64u
+
 /*
  * Parameters of one scene.  Field meaning follows
  *   SimulationParameters            src/simulation.h:10-13
@@ -176,6 +180,12 @@ int vfs_sim_autotune(vfs_sim *sim, uint32_t substeps_per_trial, vfs_tuning *best
  * the stage name (at most max_stages entries; any output array may be NULL). */
 int vfs_sim_profile_stages(vfs_sim *sim, uint32_t n, float *ms, uint32_t *launches,
                            const char **names, uint32_t max_stages, uint32_t *n_stages);
+/* Neighbour lists of the last substep: mean and largest entries per particle
+ * (in-range neighbours, the particle itself included once per visit of its
+ * own bucket) and the number of particles whose list exceeded the list
+ * capacity of VFS_NEIGHBOUR_CAPACITY entries (their forces walk the cells).
+ * No counterpart in the reference (it has no neighbour lists). */
+int vfs_sim_neighbour_stats(vfs_sim *sim, double *mean, uint32_t *max_count, uint32_t *overflowed);
 
 /* ---- compute layer drop-ins (reference src/compute headers) ------------ */
 
@@ -220,7 +230,9 @@ typedef enum vfs_shape_kind {
  * analytic shape, with the reference's domain (shape AABB grown by
  * tolerance + margin, mesh_sdf.cpp:28-45) and volume-map integrand
  * (16^3 Gauss-Legendre, volume_map.cpp:9-54).  box_size/box_pos receive the
- * grid domain.  Runs on host threads (threads <= 0: hardware concurrency). */
+ * grid domain.  Evaluated on the GPU in FP64, one grid node per thread, and
+ * copied to the host arrays; `threads` is accepted for source compatibility
+ * and ignored. */
 int vfs_volume_map_generate(int shape_kind, const double *shape_params, const uint32_t *res,
                             double support_radius, double tolerance, double margin,
                             float *sdf_out, float *vm_out, float *box_size, float *box_pos,

@@ -149,6 +149,15 @@ def oracle_volume_map(kind, shape_params, resolution, support_radius, tolerance=
                         tuple(float(v) for v in bpos))
 
 
+def hash_cells(c):
+    """HashCell3D of integer cells (m, 3) in numpy (mod 2^32), for brute-force
+    neighbour counts; the oracle's own hash is pinned by the known answers."""
+    u = (np.asarray(c, np.int64) + 25) & 0xFFFFFFFF
+    lo, hi = u % 50, u // 50
+    h = lo[:, 0] + 50 * (lo[:, 1] + 50 * lo[:, 2]) + hi[:, 0] * 15823 + hi[:, 1] * 9737333 + hi[:, 2] * 440817757
+    return h & 0xFFFFFFFF
+
+
 def oracle_keys(x, cell, table):
     x = np.ascontiguousarray(x, np.float32).reshape(-1)
     n = x.size // 3

@@ -48,3 +48,27 @@ def test_compact_lattice_seeding_like_the_reference():
     sc.step()
     x1 = sc.get_state()[0]
     assert np.isfinite(x1).all() and not np.array_equal(x0, x1)
+
+
+def test_cpp_facade_matches_the_c_abi(tmp_path):
+    """tests/cpp/facade_scene (Simulation / LoadScene / GenericScene in C++)
+    gives the same state as the ctypes scene path, bit for bit."""
+    import subprocess
+    exe = Path(__file__).resolve().parent / "cpp" / "facade_scene"
+    out = tmp_path / "state.bin"
+    r = subprocess.run([str(exe), str(SCENES / "dam_break_walls.json"), "3", str(out)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = out.read_bytes()
+    n = int(np.frombuffer(raw[:4], np.uint32)[0])
+    f = np.frombuffer(raw[4:], np.float32)
+    x, v, rho = f[:3 * n].reshape(n, 3), f[3 * n:6 * n].reshape(n, 3), f[6 * n:7 * n]
+    ids = np.frombuffer(raw[4 + 28 * n:], np.uint32)
+    sc = V.Scene(str(SCENES / "dam_break_walls.json"))
+    for _ in range(3):
+        sc.step()
+    ex, ev, er, eid = sc.get_state(ids=True)
+    np.testing.assert_array_equal(x, ex)
+    np.testing.assert_array_equal(v, ev)
+    np.testing.assert_array_equal(rho, er)
+    np.testing.assert_array_equal(ids, eid)

@@ -273,3 +273,103 @@ def test_autotune_keeps_results_and_returns_a_valid_tuning():
     assert a.get_tuning().as_tuple() == best.as_tuple()
     with pytest.raises(V.VfsError):
         a.set_tuning(7, 16, 256, 0)
+
+
+def _dense_block(counts, spacing, origin, seed):
+    return CF.lattice(counts, spacing, origin, 0.1 * spacing, seed)
+
+
+def test_neighbour_list_overflow_bit_exact():
+    """Every particle has more neighbours than the list holds (spacing 0.3 h):
+    density and forces walk the cells instead, with identical results."""
+    h = 0.05
+    x = _dense_block((10, 10, 10), 0.3 * h, (0.3, 0.3, 0.3), 5)
+    n = x.shape[0]
+    p = V.make_params(n, h, dt=1e-5, box_size=(1, 1, 1), flags=V.VFS_FLAG_TRACK_IDS)
+    sim = V.Sim(p, [])
+    sim.set_state(x)
+    orc = H.OracleSim(p, [])
+    orc.set_state(x)
+    for _ in range(3):
+        sim.substeps(1)
+        orc.substeps(1)
+        _assert_same_step(sim, orc)
+    mean, mx, over = sim.neighbour_stats()
+    assert over > n // 2 and mx > V.VFS_NEIGHBOUR_CAPACITY
+
+
+def test_neighbour_list_mixed_overflow_bit_exact():
+    """A dense cluster (lists overflow) next to a sparse block (lists fit), so
+    waves mix both paths."""
+    h = 0.05
+    dense = _dense_block((6, 6, 6), 0.3 * h, (0.2, 0.2, 0.2), 6)
+    sparse = _dense_block((12, 12, 12), 0.5 * h, (0.3, 0.2, 0.2), 7)
+    x = np.ascontiguousarray(np.concatenate([dense, sparse]))
+    n = x.shape[0]
+    p = V.make_params(n, h, dt=1e-5, box_size=(1, 1, 1), flags=V.VFS_FLAG_TRACK_IDS)
+    sim = V.Sim(p, [])
+    sim.set_state(x)
+    orc = H.OracleSim(p, [])
+    orc.set_state(x)
+    for _ in range(3):
+        sim.substeps(1)
+        orc.substeps(1)
+        _assert_same_step(sim, orc)
+    _, _, over = sim.neighbour_stats()
+    assert 0 < over < n
+
+
+def test_neighbour_stats_match_brute_force():
+    """List lengths = in-range particles over the 27 buckets (self included),
+    counted on the CPU from the same sorted tables."""
+    w = CF.dam_break(side=(16, 16, 16), map_res=32)
+    sim, orc = _pair(w, _grids(w))
+    sim.substeps(1)
+    orc.substeps(1)
+    keys, perm, offsets = orc.get_hash()
+    xs = w.x[perm].astype(np.float64)
+    n = w.n
+    cell = np.floor(xs / np.float64(np.float32(w.h))).astype(np.int64)
+    starts = np.searchsorted(keys, np.arange(n + 1), side="left")
+    counts = np.zeros(n, np.int64)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                k = H.hash_cells(cell + np.array([dx, dy, dz])) % n
+                for i in range(n):
+                    js = np.arange(starts[k[i]], starts[k[i] + 1])
+                    d = xs[js] - xs[i]
+                    counts[i] += int(np.count_nonzero((d * d).sum(axis=1) <= np.float64(np.float32(w.h)) ** 2))
+    mean, mx, over = sim.neighbour_stats()
+    assert over == 0
+    assert abs(mean - counts.mean()) <= 1e-3 * counts.mean()
+    assert abs(mx - counts.max()) <= 1
+
+
+def test_fp64_drift_dam_break_100_substeps():
+    """Drift of the FP32 GPU state against the FP64 restatement after 100
+    substeps of the 32,768-particle dam break, per field (max |difference| over
+    the field's max magnitude, and the same at the 99th percentile).  FP32
+    rounding grows through the chaotic flow; the bounds are a sanity check and
+    the printed figures are the report."""
+    w = CF.dam_break()
+    grids = _grids(w)
+    p = _params(w)
+    sim = V.Sim(p, grids)
+    sim.set_state(w.x, w.v)
+    d = H.OracleSim(p, grids, fp64=True)
+    d.set_state(w.x, w.v)
+    sim.substeps(w.steps)
+    d.substeps(w.steps)
+    x, v, rho, pid = sim.get_state(ids=True)
+    dx, dv, drho, dpid, _ = d.get_state()
+    report = {}
+    for name, a, b in (("x", x, dx), ("v", v, dv), ("rho", rho, drho)):
+        a = H.by_id(a, pid).astype(np.float64)
+        b = H.by_id(b, dpid)
+        diff = np.abs(a - b).reshape(len(a), -1).max(axis=1)
+        scale = float(np.abs(b).max())
+        report[name] = (float(diff.max()) / scale, float(np.percentile(diff, 99)) / scale)
+    print("FP32 (GPU) vs FP64 drift after 100 substeps, max / p99 relative to field scale: " +
+          ", ".join(f"{k} {m:.3e} / {q:.3e}" for k, (m, q) in report.items()))
+    assert report["x"][0] < 1e-3 and report["rho"][1] < 1e-2

@@ -72,3 +72,15 @@ def test_scene_loader_errors(name, code, needle):
     rc, msg = _err(V.lib.vfs_scene_load, str(SCENES / name).encode(), b"", C.byref(out))
     assert rc == code and needle in msg
     assert out.value is None
+
+
+def test_cpp_facade_example_links():
+    """The C++ facade example links against libvfs.so and starts (usage path,
+    no device work)."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent / "cpp" / "facade_scene"
+    if not exe.exists():
+        subprocess.run(["make", "-C", str(exe.parent)], check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr

@@ -391,3 +391,11 @@ def test_volume_map_half_space_against_1d_integral():
         assert abs(got - bias * ref) <= 0.01 * full
         seen += 1
     assert seen >= 3
+
+
+def test_numpy_hash_matches_oracle():
+    rng = np.random.default_rng(3)
+    c = rng.integers(-200, 200, size=(500, 3))
+    c[:3] = [[-25, -25, -25], [24, 24, 24], [25, 25, 25]]
+    want = np.array([H.ORC.orc_hash_cell(int(a), int(b), int(d)) for a, b, d in c], np.int64)
+    np.testing.assert_array_equal(H.hash_cells(c), want)

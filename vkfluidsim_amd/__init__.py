@@ -13,11 +13,11 @@ import numpy as np
 
 from . import _lib
 from ._lib import (BoundaryObject, DeviceArray, Params, Tuning, VfsError, VFS_FLAG_NO_GRAPH, VFS_FLAG_STORE_ACCEL,
-                   VFS_FLAG_TRACK_IDS, VFS_SHAPE_BOX, VFS_SHAPE_BOX_WALLS, VFS_SHAPE_SPHERE, call, fptr, lib, uptr)
+                   VFS_FLAG_TRACK_IDS, VFS_NEIGHBOUR_CAPACITY, VFS_SHAPE_BOX, VFS_SHAPE_BOX_WALLS, VFS_SHAPE_SPHERE, call, fptr, lib, uptr)
 
 __all__ = ["Sim", "Params", "Tuning", "BoundaryGrid", "SpatialHash", "VfsError", "make_params", "volume_map",
            "sort_pairs", "scan_exclusive", "spatial_offsets", "reorder_f32x3", "Scene", "lib",
-           "VFS_FLAG_TRACK_IDS", "VFS_FLAG_NO_GRAPH", "VFS_FLAG_STORE_ACCEL",
+           "VFS_FLAG_TRACK_IDS", "VFS_FLAG_NO_GRAPH", "VFS_FLAG_STORE_ACCEL", "VFS_NEIGHBOUR_CAPACITY",
            "VFS_SHAPE_BOX", "VFS_SHAPE_SPHERE", "VFS_SHAPE_BOX_WALLS", "device_count"]
 
 device_count = _lib.device_count
@@ -179,6 +179,12 @@ class Sim:
         cnt = C.c_uint32(0)
         call("vfs_sim_profile_stages", self.handle, int(k), ms, la, names, m, C.byref(cnt))
         return [(names[i].decode(), float(ms[i]), int(la[i])) for i in range(cnt.value)]
+
+    def neighbour_stats(self):
+        """(mean, max, overflowed) list entries per particle of the last substep."""
+        mean, mx, over = C.c_double(0), C.c_uint32(0), C.c_uint32(0)
+        call("vfs_sim_neighbour_stats", self.handle, C.byref(mean), C.byref(mx), C.byref(over))
+        return mean.value, mx.value, over.value
 
     def close(self) -> None:
         if getattr(self, "handle", None):

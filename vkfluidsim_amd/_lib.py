@@ -17,6 +17,7 @@ LIB_PATH = PKG_DIR / "libvfs.so"
 VFS_FLAG_TRACK_IDS = 0x1
 VFS_FLAG_NO_GRAPH = 0x2
 VFS_FLAG_STORE_ACCEL = 0x4
+VFS_NEIGHBOUR_CAPACITY = 64
 
 VFS_SHAPE_BOX = 0
 VFS_SHAPE_SPHERE = 1
@@ -130,6 +131,7 @@ SIGNATURES = {
     "vfs_sim_set_tuning": (_i32, [_vp, C.POINTER(Tuning)]),
     "vfs_sim_autotune": (_i32, [_vp, _u32, C.POINTER(Tuning), C.POINTER(C.c_float)]),
     "vfs_sim_profile_stages": (_i32, [_vp, _u32, _fp, _up, C.POINTER(C.c_char_p), _u32, _up]),
+    "vfs_sim_neighbour_stats": (_i32, [_vp, C.POINTER(C.c_double), _up, _up]),
     "vfs_spatial_hash_create": (_i32, [_u32, C.c_float, C.POINTER(_vp)]),
     "vfs_spatial_hash_set_cell_size": (_i32, [_vp, C.c_float]),
     "vfs_spatial_hash_run": (_i32, [_vp, _vp, _u32, _vp]),

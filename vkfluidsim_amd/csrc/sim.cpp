@@ -726,4 +726,26 @@ int vfs_sim_profile_stages(vfs_sim* sim, uint32_t n, float* ms, uint32_t* launch
     return 0;
 }
 
+int vfs_sim_neighbour_stats(vfs_sim* sim, double* mean, uint32_t* max_count, uint32_t* overflowed) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    if (mean) *mean = 0.0;
+    if (max_count) *max_count = 0;
+    if (overflowed) *overflowed = 0;
+    if (sim->n == 0 || !sim->stepped) return 0;
+    std::vector<uint32_t> counts(sim->n);
+    VFS_HIP(hipMemcpyAsync(counts.data(), sim->nbr_count, (size_t)sim->n * 4, hipMemcpyDeviceToHost, sim->stream));
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    uint64_t sum = 0;
+    uint32_t mx = 0, over = 0;
+    for (uint32_t c : counts) {
+        sum += c;
+        mx = std::max(mx, c);
+        over += c > kNeighbourCap ? 1u : 0u;
+    }
+    if (mean) *mean = (double)sum / sim->n;
+    if (max_count) *max_count = mx;
+    if (overflowed) *overflowed = over;
+    return 0;
+}
+
 }  // extern "C"

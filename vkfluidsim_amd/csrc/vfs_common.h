@@ -98,7 +98,8 @@ struct SimConsts {
 
 // Neighbour-list slots per particle, a multiple of 4 (the lattice workloads
 // have ~30 neighbours).
-constexpr uint32_t kNeighbourCap = 64;
+constexpr uint32_t kNeighbourCap = VFS_NEIGHBOUR_CAPACITY;
+static_assert(kNeighbourCap % 4 == 0, "lists are stored in chunks of four");
 
 // Boundary object as the kernels see it (column-major matrices).
 struct DevObject {
