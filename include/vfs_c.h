@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define VFS_ABI_VERSION 1
+#define VFS_ABI_VERSION 2
 
 typedef enum vfs_status {
     VFS_OK = 0,
@@ -101,16 +101,21 @@ typedef struct vfs_device_buffers {
     uint32_t n_particles;
 } vfs_device_buffers;
 
-/* Launch configuration of a simulation.  Every setting produces bit-identical
- * results; only the speed changes. */
+/* Launch configuration of a simulation.  With cell_scale = 1 every setting
+ * produces bit-identical results; only the speed changes.  cell_scale > 1
+ * hashes with cells that many times the scene's own cell size: the same
+ * neighbours are found, but summed in another order (no bit parity with the
+ * reference hash). */
 typedef struct vfs_tuning {
     int32_t sort_bits;   /* radix digit width of the key sort: 8, 9 or 10 */
     int32_t sort_items;  /* keys per thread of a sort tile: 8, 12 or 16 */
     int32_t block_size;  /* threads per block of the neighbour kernels: 128 or 256 */
     int32_t stage_rows;  /* 1: stage stencil rows of positions in LDS; 0: read them through L1/L2 */
+    float cell_scale;    /* hash cell size / the scene's cell size, in [1, 4]; 1 = reference hash */
 } vfs_tuning;
 
 typedef struct vfs_sim vfs_sim;
+typedef struct vfs_batch vfs_batch;
 typedef struct vfs_spatial_hash vfs_spatial_hash;
 
 /* ---- library ---------------------------------------------------------- */
@@ -186,6 +191,41 @@ int vfs_sim_profile_stages(vfs_sim *sim, uint32_t n, float *ms, uint32_t *launch
  * capacity of VFS_NEIGHBOUR_CAPACITY entries (their forces walk the cells).
  * No counterpart in the reference (it has no neighbour lists). */
 int vfs_sim_neighbour_stats(vfs_sim *sim, double *mean, uint32_t *max_count, uint32_t *overflowed);
+
+/* ---- batches of independent scenes ------------------------------------ */
+/* No counterpart in the reference (one scene per Simulation, src/simulation.h:49).
+ * A batch steps n_scenes scenes of uneven sizes together: one launch per
+ * stage serves every scene, the substep is one hipGraph, and each scene
+ * evolves exactly as it would alone in a vfs_sim (bit-identical).  Scene k
+ * has params[k] (all with the same flags) and objects_per_scene[k] boundary
+ * objects, listed scene after scene in `objects`.  Scenes are padded to 64
+ * entries inside the batch; the total must stay below 2^28 entries. */
+int vfs_batch_create(const vfs_params *params, uint32_t n_scenes, const vfs_boundary_object *objects,
+                     const uint32_t *objects_per_scene, vfs_batch **out);
+int vfs_batch_destroy(vfs_batch *batch);
+/* Scene count, particles over all scenes, and entries per array (with padding). */
+int vfs_batch_info(const vfs_batch *batch, uint32_t *n_scenes, uint64_t *n_particles, uint32_t *entries);
+/* Per-scene versions of the vfs_sim calls (AoS float3 host arrays of that scene). */
+int vfs_batch_set_state(vfs_batch *batch, uint32_t scene, const float *positions, const float *velocities);
+int vfs_batch_set_params(vfs_batch *batch, uint32_t scene, const vfs_params *params);
+/* n substeps of every scene, each at its own dt. */
+int vfs_batch_substeps(vfs_batch *batch, uint32_t n, void *stream);
+int vfs_batch_synchronize(vfs_batch *batch);
+int vfs_batch_get_state(vfs_batch *batch, uint32_t scene, float *positions, float *velocities,
+                        float *densities, uint32_t *particle_ids);
+/* The scene's hash tables in its own index space (as vfs_sim_get_hash). */
+int vfs_batch_get_hash(vfs_batch *batch, uint32_t scene, uint32_t *sorted_keys, uint32_t *sorted_indices,
+                       uint32_t *offsets);
+int vfs_batch_get_boundary(vfs_batch *batch, uint32_t scene, float *boundary_density, float *boundary_gradient);
+int vfs_batch_get_tuning(const vfs_batch *batch, vfs_tuning *tuning);
+int vfs_batch_set_tuning(vfs_batch *batch, const vfs_tuning *tuning);
+/* As vfs_sim_autotune; with allow_cell_size != 0 it also tries cell scales
+ * 1.25 and 1.5 (results then differ from the reference hash's in rounding). */
+int vfs_batch_autotune(vfs_batch *batch, uint32_t substeps_per_trial, int allow_cell_size, vfs_tuning *best,
+                       float *best_ms_per_substep);
+int vfs_batch_profile_stages(vfs_batch *batch, uint32_t n, float *ms, uint32_t *launches,
+                             const char **names, uint32_t max_stages, uint32_t *n_stages);
+int vfs_batch_neighbour_stats(vfs_batch *batch, double *mean, uint32_t *max_count, uint32_t *overflowed);
 
 /* ---- compute layer drop-ins (reference src/compute headers) ------------ */
 

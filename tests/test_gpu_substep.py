@@ -235,7 +235,7 @@ def test_profile_stages_report_every_kernel():
     assert dict((s[0], s[2]) for s in stages)["sort_pass"] == 2  # 12-bit keys at n = 4096
 
 
-TUNINGS = [(b, it, bs, st) for b in (8, 9, 10) for it in (8, 12, 16) for bs in (128, 256) for st in (0, 1)]
+TUNINGS = [(b, it, bs, st, 1.0) for b in (8, 9, 10) for it in (8, 12, 16) for bs in (128, 256) for st in (0, 1)]
 
 
 def test_every_tuning_is_bit_identical():
@@ -273,6 +273,30 @@ def test_autotune_keeps_results_and_returns_a_valid_tuning():
     assert a.get_tuning().as_tuple() == best.as_tuple()
     with pytest.raises(V.VfsError):
         a.set_tuning(7, 16, 256, 0)
+    with pytest.raises(V.VfsError):
+        a.set_tuning(8, 16, 256, 0, 0.5)
+
+
+def test_larger_hash_cells_find_the_same_neighbours():
+    """cell_scale > 1 hashes coarser cells: the same in-range pairs, summed in
+    another order, so the state agrees with the reference hash to rounding."""
+    w = CF.dam_break(side=(16, 16, 16), map_res=32)
+    grids = _grids(w)
+    a = V.Sim(_params(w), grids)
+    a.set_tuning(8, 16, 128, 1, 1.5)
+    a.set_state(w.x, w.v)
+    a.substeps(6)
+    orc = H.OracleSim(_params(w), grids)
+    orc.set_state(w.x, w.v)
+    orc.substeps(6)
+    x, v, rho, pid = a.get_state(ids=True)
+    rx, rv, rrho, rpid, _ = orc.get_state()
+    assert H.rel_err(H.by_id(x, pid), H.by_id(rx, rpid)) < 1e-6
+    assert H.rel_err(H.by_id(v, pid), H.by_id(rv, rpid)) < 1e-4
+    assert H.rel_err(H.by_id(rho, pid), H.by_id(rrho, rpid)) < 1e-5
+    k, _, _ = a.get_hash()
+    rk, _, _ = orc.get_hash()
+    assert not np.array_equal(k, rk)
 
 
 def _dense_block(counts, spacing, origin, seed):

@@ -22,7 +22,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_abi_version_and_device_count():
-    assert V.lib.vfs_abi_version() == 1
+    assert V.lib.vfs_abi_version() == 2
     assert V.device_count() >= 0
 
 

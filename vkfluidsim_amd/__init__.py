@@ -15,7 +15,7 @@ from . import _lib
 from ._lib import (BoundaryObject, DeviceArray, Params, Tuning, VfsError, VFS_FLAG_NO_GRAPH, VFS_FLAG_STORE_ACCEL,
                    VFS_FLAG_TRACK_IDS, VFS_NEIGHBOUR_CAPACITY, VFS_SHAPE_BOX, VFS_SHAPE_BOX_WALLS, VFS_SHAPE_SPHERE, call, fptr, lib, uptr)
 
-__all__ = ["Sim", "Params", "Tuning", "BoundaryGrid", "SpatialHash", "VfsError", "make_params", "volume_map",
+__all__ = ["Sim", "Batch", "Params", "Tuning", "BoundaryGrid", "SpatialHash", "VfsError", "make_params", "volume_map",
            "sort_pairs", "scan_exclusive", "spatial_offsets", "reorder_f32x3", "Scene", "lib",
            "VFS_FLAG_TRACK_IDS", "VFS_FLAG_NO_GRAPH", "VFS_FLAG_STORE_ACCEL", "VFS_NEIGHBOUR_CAPACITY",
            "VFS_SHAPE_BOX", "VFS_SHAPE_SPHERE", "VFS_SHAPE_BOX_WALLS", "device_count"]
@@ -154,8 +154,8 @@ class Sim:
         call("vfs_sim_get_tuning", self.handle, C.byref(t))
         return t
 
-    def set_tuning(self, sort_bits=8, sort_items=16, block_size=256, stage_rows=0) -> None:
-        t = Tuning(sort_bits, sort_items, block_size, stage_rows)
+    def set_tuning(self, sort_bits=8, sort_items=16, block_size=128, stage_rows=1, cell_scale=1.0) -> None:
+        t = Tuning(sort_bits, sort_items, block_size, stage_rows, cell_scale)
         call("vfs_sim_set_tuning", self.handle, C.byref(t))
 
     def autotune(self, substeps_per_trial: int = 5):
@@ -288,6 +288,108 @@ def reorder_f32x3(perm: np.ndarray, src: np.ndarray) -> np.ndarray:
     call("vfs_reorder_f32x3", dp.ptr, ds.ptr, dd.ptr, n, None)
     call("vfs_stream_synchronize", None)
     return dd.to_host(np.float32, 3 * n).reshape(n, 3)
+
+
+class Batch:
+    """Independent scenes stepped together (vfs_batch_*): scene k has params[k]
+    and the boundary grids objects[k]."""
+
+    def __init__(self, params: list, objects: list | None = None):
+        self.n_scenes = len(params)
+        objects = objects or [[] for _ in params]
+        assert len(objects) == self.n_scenes
+        self.params = list(params)
+        self.sizes = [int(p.n_particles) for p in params]
+        self._objects = [list(o) for o in objects]
+        flat = [g for objs in self._objects for g in objs]
+        arr = (BoundaryObject * max(1, len(flat)))()
+        for k, g in enumerate(flat):
+            arr[k] = g.as_struct()
+        pa = (Params * self.n_scenes)(*params)
+        counts = np.array([len(o) for o in self._objects], np.uint32)
+        h = C.c_void_p()
+        call("vfs_batch_create", pa, self.n_scenes, arr if flat else None, uptr(counts), C.byref(h))
+        self.handle = h.value
+
+    def info(self):
+        ns, npart, entries = C.c_uint32(0), C.c_uint64(0), C.c_uint32(0)
+        call("vfs_batch_info", self.handle, C.byref(ns), C.byref(npart), C.byref(entries))
+        return ns.value, npart.value, entries.value
+
+    def set_state(self, scene: int, x: np.ndarray, v: np.ndarray | None = None) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+        v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, dtype=np.float32).reshape(-1)
+        call("vfs_batch_set_state", self.handle, int(scene), fptr(x), fptr(v))
+
+    def substeps(self, k: int, stream=None) -> None:
+        call("vfs_batch_substeps", self.handle, int(k), stream)
+
+    def synchronize(self) -> None:
+        call("vfs_batch_synchronize", self.handle)
+
+    def get_state(self, scene: int, ids: bool = False):
+        n = self.sizes[scene]
+        x = np.empty(3 * n, np.float32)
+        v = np.empty(3 * n, np.float32)
+        rho = np.empty(n, np.float32)
+        pid = np.empty(n, np.uint32) if ids else None
+        call("vfs_batch_get_state", self.handle, int(scene), fptr(x), fptr(v), fptr(rho),
+             uptr(pid) if ids else None)
+        return x.reshape(n, 3), v.reshape(n, 3), rho, pid
+
+    def get_hash(self, scene: int):
+        n = self.sizes[scene]
+        k, p, o = (np.empty(n, np.uint32) for _ in range(3))
+        call("vfs_batch_get_hash", self.handle, int(scene), uptr(k), uptr(p), uptr(o))
+        return k, p, o
+
+    def get_boundary(self, scene: int):
+        n = self.sizes[scene]
+        psi = np.empty(n, np.float32)
+        g = np.empty(3 * n, np.float32)
+        call("vfs_batch_get_boundary", self.handle, int(scene), fptr(psi), fptr(g))
+        return psi, g.reshape(n, 3)
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        call("vfs_batch_get_tuning", self.handle, C.byref(t))
+        return t
+
+    def set_tuning(self, sort_bits=8, sort_items=16, block_size=128, stage_rows=1, cell_scale=1.0) -> None:
+        t = Tuning(sort_bits, sort_items, block_size, stage_rows, cell_scale)
+        call("vfs_batch_set_tuning", self.handle, C.byref(t))
+
+    def autotune(self, substeps_per_trial: int = 5, allow_cell_size: bool = False):
+        t = Tuning()
+        ms = C.c_float(0)
+        call("vfs_batch_autotune", self.handle, int(substeps_per_trial), 1 if allow_cell_size else 0, C.byref(t),
+             C.byref(ms))
+        return t, ms.value
+
+    def profile(self, k: int):
+        m = 16
+        ms = (C.c_float * m)()
+        la = (C.c_uint32 * m)()
+        names = (C.c_char_p * m)()
+        cnt = C.c_uint32(0)
+        call("vfs_batch_profile_stages", self.handle, int(k), ms, la, names, m, C.byref(cnt))
+        return [(names[i].decode(), float(ms[i]), int(la[i])) for i in range(cnt.value)]
+
+    def neighbour_stats(self):
+        mean, mx, over = C.c_double(0), C.c_uint32(0), C.c_uint32(0)
+        call("vfs_batch_neighbour_stats", self.handle, C.byref(mean), C.byref(mx), C.byref(over))
+        return mean.value, mx.value, over.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.vfs_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Scene:

@@ -71,10 +71,10 @@ class BoundaryObject(C.Structure):
 class Tuning(C.Structure):
     """vfs_tuning: launch configuration (bit-identical results for every setting)."""
     _fields_ = [("sort_bits", C.c_int32), ("sort_items", C.c_int32), ("block_size", C.c_int32),
-                ("stage_rows", C.c_int32)]
+                ("stage_rows", C.c_int32), ("cell_scale", C.c_float)]
 
     def as_tuple(self):
-        return (self.sort_bits, self.sort_items, self.block_size, self.stage_rows)
+        return (self.sort_bits, self.sort_items, self.block_size, self.stage_rows, self.cell_scale)
 
 
 class DeviceBuffers(C.Structure):
@@ -132,6 +132,21 @@ SIGNATURES = {
     "vfs_sim_autotune": (_i32, [_vp, _u32, C.POINTER(Tuning), C.POINTER(C.c_float)]),
     "vfs_sim_profile_stages": (_i32, [_vp, _u32, _fp, _up, C.POINTER(C.c_char_p), _u32, _up]),
     "vfs_sim_neighbour_stats": (_i32, [_vp, C.POINTER(C.c_double), _up, _up]),
+    "vfs_batch_create": (_i32, [_vp, _u32, _vp, _up, C.POINTER(_vp)]),
+    "vfs_batch_destroy": (_i32, [_vp]),
+    "vfs_batch_info": (_i32, [_vp, _up, C.POINTER(C.c_uint64), _up]),
+    "vfs_batch_set_state": (_i32, [_vp, _u32, _fp, _fp]),
+    "vfs_batch_set_params": (_i32, [_vp, _u32, _vp]),
+    "vfs_batch_substeps": (_i32, [_vp, _u32, _vp]),
+    "vfs_batch_synchronize": (_i32, [_vp]),
+    "vfs_batch_get_state": (_i32, [_vp, _u32, _fp, _fp, _fp, _up]),
+    "vfs_batch_get_hash": (_i32, [_vp, _u32, _up, _up, _up]),
+    "vfs_batch_get_boundary": (_i32, [_vp, _u32, _fp, _fp]),
+    "vfs_batch_get_tuning": (_i32, [_vp, C.POINTER(Tuning)]),
+    "vfs_batch_set_tuning": (_i32, [_vp, C.POINTER(Tuning)]),
+    "vfs_batch_autotune": (_i32, [_vp, _u32, _i32, C.POINTER(Tuning), C.POINTER(C.c_float)]),
+    "vfs_batch_profile_stages": (_i32, [_vp, _u32, _fp, _up, C.POINTER(C.c_char_p), _u32, _up]),
+    "vfs_batch_neighbour_stats": (_i32, [_vp, C.POINTER(C.c_double), _up, _up]),
     "vfs_spatial_hash_create": (_i32, [_u32, C.c_float, C.POINTER(_vp)]),
     "vfs_spatial_hash_set_cell_size": (_i32, [_vp, C.c_float]),
     "vfs_spatial_hash_run": (_i32, [_vp, _vp, _u32, _vp]),

@@ -117,6 +117,34 @@ def uniform_stress(n: int = 8_388_608, h: float = 0.02, map_res: int = 256, step
     return Workload("wcsph_uniform_stress", n, h, 1e-4, steps, (4.0, 2.0, 2.0), (0.0, 0.0, 0.0), x, v, [walls])
 
 
+def batched_dam_breaks(n_scenes: int = 256, seed: int = SEED_BASE + 3, map_res: int = 32, steps: int = 200,
+                       n_min: int = 2048, n_max: int = 65536) -> list:
+    """Independent dam breaks of uneven sizes: n_s = round(n_min (n_max/n_min)^u)
+    (u uniform), box dimensions uniform in [0.5, 2] per axis, spacing
+    d_s = cbrt(V_box / (3 n_s)) shrunk by 1% steps until the left third of the
+    box holds n_s lattice points, h_s = 2 d_s, jitter 0.05 d_s, dt = 2e-4, box
+    walls as a map_res^3 inverted-box volume map.  Returns a list of Workload."""
+    u = uniform(seed, 4 * n_scenes).astype(np.float64).reshape(n_scenes, 4)
+    out = []
+    for s in range(n_scenes):
+        n = int(round(n_min * (n_max / n_min) ** u[s, 0]))
+        box = tuple(float(np.float32(0.5 + 1.5 * u[s, a])) for a in (1, 2, 3))
+        d = float(np.cbrt(box[0] * box[1] * box[2] / (3.0 * n)))
+        while True:
+            cx = int((box[0] / 3.0) / d)
+            cy = int((box[1] - 2.0 * d) / d)
+            cz = int((box[2] - 2.0 * d) / d)
+            if cx * cy * cz >= n:
+                break
+            d *= 0.99
+        x = lattice((cx, cy, cz), d, (d, d, d), 0.05 * d, seed + 1 + s)[:n]
+        h = 2.0 * d
+        walls = (2, (box[0] / 2, box[1] / 2, box[2] / 2, box[0] / 2, box[1] / 2, box[2] / 2), (map_res,) * 3)
+        out.append(Workload(f"dam_break_batch_{s}", n, h, 2e-4, steps, box, (0.0, 0.0, 0.0),
+                            np.ascontiguousarray(x), np.zeros_like(x), [walls]))
+    return out
+
+
 WORKLOADS = {
     "dam_break_32k": lambda: dam_break(),
     "obstacle_1m": lambda: obstacle_1m(),

@@ -63,7 +63,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 
 // UpdateSpatialHash: key = HashCell3D(floor(x / cell)) % table_size.
 __global__ __launch_bounds__(256) void k_hash_keys(const float* __restrict__ pos, uint32_t stride,
-                                                   uint32_t n, float cell, FastMod fm,
+                                                   uint32_t n, float cell, FastMod fm, uint32_t key_base,
                                                    uint32_t* __restrict__ keys) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_hash_keys(const float* __restrict__ pos
     const int32_t cx = floor_to_cell(p[0] / cell);
     const int32_t cy = floor_to_cell(p[1] / cell);
     const int32_t cz = floor_to_cell(p[2] / cell);
-    keys[i] = fastmod(hash_cell(cx, cy, cz), fm);
+    keys[i] = key_base + fastmod(hash_cell(cx, cy, cz), fm);
 }
 
 __global__ __launch_bounds__(256) void k_zero_words(uint32_t* __restrict__ a, size_t n) {
@@ -569,10 +569,10 @@ int sort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_ou
 }
 
 int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,
-                     uint32_t* keys, hipStream_t s) {
+                     uint32_t* keys, hipStream_t s, uint32_t key_base) {
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_hash_keys, dim3(grid_for(n)), dim3(256), 0, s, pos, stride, n, cell,
-                       make_fastmod(table_size), keys);
+                       make_fastmod(table_size), key_base, keys);
     return check_launch("hash_keys");
 }
 

@@ -46,8 +46,9 @@ int sort_keys_iota(const SortPlan& plan, const uint32_t* keys_in, uint32_t* keys
                    uint32_t* workspace, hipStream_t s, StageTimer* timer, uint32_t* fill = nullptr,
                    size_t fill_words = 0, uint32_t fill_value = 0);
 
+// keys[i] = key_base + HashCell3D(floor(pos_i / cell)) % table_size
 int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,
-                     uint32_t* keys, hipStream_t s);
+                     uint32_t* keys, hipStream_t s, uint32_t key_base = 0);
 // table[sorted_keys[i]] = i where a key starts (CalculateOffsets).
 int launch_key_starts(const uint32_t* sorted_keys, uint32_t n, uint32_t* table, hipStream_t s);
 int launch_fill_value(uint32_t* a, size_t count, uint32_t value, hipStream_t s);
@@ -82,7 +83,7 @@ struct StepBuffers {
     uint32_t* nbr;       // nbr_cap * n: neighbour lists of the substep in chunks of four,
                          // chunk-major (uint4 chunk g of particle i at g * n + i)
     uint32_t* nbr_count; // n: list lengths (> nbr_cap: overflowed)
-    uint32_t n;
+    uint32_t n;          // entries of every per-particle array (all scenes of a batch, padded)
 };
 
 // Candidates per group of the density pass's cell walk (its last group reads
@@ -91,9 +92,15 @@ constexpr uint32_t kCandidateGroup = 4;
 
 int launch_reorder_state(const StepBuffers& b, hipStream_t s);
 // block: 128 or 256 threads; staged: stage stencil rows of positions in LDS.
-int launch_density(const StepBuffers& b, const SimConsts& c, const DevObject* objs, uint32_t n_obj, int block,
-                   int staged, hipStream_t s);
-int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, int block, int staged, hipStream_t s);
+// table == nullptr: one scene with constants c over all b.n entries; else a
+// batch whose waves read their scene's constants from the table (c unused).
+int launch_density(const StepBuffers& b, const SimConsts& c, const SceneTable* table, const DevObject* objs,
+                   int block, int staged, hipStream_t s);
+int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, const SceneTable* table, int block,
+                            hipStream_t s);
+// Waves of a padded particle launch of n entries (blocks of up to 256 threads,
+// rounded to a multiple of 8 blocks): the size of a batch's wave_scene table.
+inline uint32_t launch_waves(uint32_t n) { return ((((n + 255u) / 256u) + 7u) & ~7u) * 4u; }
 
 // Volume-map/SDF grid generator for analytic shapes (FP64 on the device).
 int launch_volume_map(int kind, const double* shape_params, const uint32_t* res, double support_radius,

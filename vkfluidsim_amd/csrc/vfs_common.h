@@ -91,9 +91,21 @@ struct SimConsts {
     float visc_eps_h2;     // 0.01 * h^2
     float rho0_sq;         // rho0 * rho0
     float bb_lo[3], bb_hi[3];
-    uint32_t n;
+    uint32_t n;            // particles of the scene
     uint32_t nbr_cap;      // neighbour-list slots per particle
     FastMod fm;            // % n
+    // Scene segment inside the particle arrays (batched scenes; 0 for one
+    // scene): the scene's particles are entries [base, base + n) and its hash
+    // keys are base + (hash % n).
+    uint32_t base;
+    uint32_t obj_begin, obj_count;  // its boundary objects in the object array
+};
+
+// Per-wave scene lookup of batched launches: a batch pads every scene to a
+// multiple of 64 entries so that a wave never spans two scenes.
+struct SceneTable {
+    const SimConsts* consts;      // [n_scenes + 1]; the last entry has n = 0
+    const uint32_t* wave_scene;   // scene of each wave of the padded launch
 };
 
 // Neighbour-list slots per particle, a multiple of 4 (the lattice workloads

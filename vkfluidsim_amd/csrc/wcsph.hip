@@ -77,9 +77,20 @@ __device__ __forceinline__ float eos(float rho, const SimConsts& c) {
 
 __device__ __forceinline__ uint32_t cell_key(int32_t cx, int32_t cy, int32_t cz, int dx, int dy, int dz,
                                              const SimConsts& c) {
-    return fastmod(hash_cell((int32_t)((uint32_t)cx + (uint32_t)dx), (int32_t)((uint32_t)cy + (uint32_t)dy),
-                             (int32_t)((uint32_t)cz + (uint32_t)dz)),
-                   c.fm);
+    return c.base + fastmod(hash_cell((int32_t)((uint32_t)cx + (uint32_t)dx),
+                                      (int32_t)((uint32_t)cy + (uint32_t)dy),
+                                      (int32_t)((uint32_t)cz + (uint32_t)dz)),
+                            c.fm);
+}
+
+// Constants of the scene of entry i: the launch argument for a single scene,
+// the wave's row of the scene table for a batch (i >> 6 is uniform over the
+// wave, so the row is read with scalar loads).
+template <bool BATCH>
+__device__ __forceinline__ SimConsts scene_consts(const SimConsts& c0, const SceneTable& t, uint32_t i) {
+    if (!BATCH) return c0;
+    const uint32_t s = __builtin_amdgcn_readfirstlane(t.wave_scene[i >> 6]);
+    return t.consts[s];
 }
 
 // SampleGrid, slang:51-66, base cell clamped to [0, n-2].
@@ -363,25 +374,26 @@ __global__ __launch_bounds__(256) void k_reorder_state(StepBuffers b, uint32_t n
 // sum then walks the particle's own list, every lane busy until its own count.
 // A particle with more than nbr_cap neighbours stores its count only; its
 // density, and its forces in the force pass, walk the cells again.
-template <int BLOCK, bool STAGED>
-__global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, const DevObject* __restrict__ objs,
-                                                   uint32_t n_obj, uint32_t nblocks) {
+template <int BLOCK, bool STAGED, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, SceneTable table,
+                                                   const DevObject* __restrict__ objs, uint32_t nblocks) {
     __shared__ uint32_t s_j[kQueueCap][BLOCK];
     // wave slices of staged rows, padded for the masked tail group's reads
     __shared__ float4 s_stage[STAGED ? BLOCK / 64 : 1][kWaveStage + kUnroll];
     const int tid = threadIdx.x;
     const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + tid;
-    const bool active = i < c.n;
+    const SimConsts c = scene_consts<BATCH>(c0, table, i);
+    const bool active = i - c.base < c.n;
     const float4 p4 = b.xs[active ? i : 0u];
     const float pos[3] = {p4.x, p4.y, p4.z};
     float psi = 0.0f, g[3] = {0.0f, 0.0f, 0.0f};
-    if (active) boundary_volume(objs, n_obj, pos, c, psi, g);
+    if (active) boundary_volume(objs + c.obj_begin, c.obj_count, pos, c, psi, g);
 
     const int32_t cx = floor_to_cell(pos[0] / c.cell);
     const int32_t cy = floor_to_cell(pos[1] / c.cell);
     const int32_t cz = floor_to_cell(pos[2] / c.cell);
     const float4* __restrict__ xs = b.xs;
-    const auto xr = make_rsrc(xs, c.n * 16u);
+    const auto xr = make_rsrc(xs, b.n * 16u);
     uint4* __restrict__ list = reinterpret_cast<uint4*>(b.nbr);
     uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // the list chunk being filled
     int cnt = 0;
@@ -401,15 +413,18 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, c
             e1 = k == 1u ? j : e1;
             e2 = k == 2u ? j : e2;
             e3 = k == 3u ? j : e3;
-            if (k == 3u && listed < c.nbr_cap) list[(size_t)(listed >> 2) * c.n + i] = make_uint4(e0, e1, e2, e3);
+            if (k == 3u && listed < c.nbr_cap) list[(size_t)(listed >> 2) * b.n + i] = make_uint4(e0, e1, e2, e3);
             listed++;
         }
         cnt = 0;
     };
+    // inactive lanes (past the scene, or past the last scene of a batch) read
+    // no cell starts: their rows are empty
     auto cells = [&](auto&& visit) {
         for (int dz = -1; dz <= 1; dz++)
             for (int dy = -1; dy <= 1; dy++) {
-                const RowRanges r = row_ranges(cx, cy, cz, dy, dz, c, b.cell_start);
+                const RowRanges r = active ? row_ranges(cx, cy, cz, dy, dz, c, b.cell_start)
+                                           : RowRanges{0u, 0u, 0u, 0u, 0u, 0u, 1};
                 visit(r);
             }
     };
@@ -435,7 +450,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, c
     });
     if (!active) return;
     flush();
-    if ((listed & 3u) != 0u && listed <= c.nbr_cap) list[(size_t)(listed >> 2) * c.n + i] = make_uint4(e0, e1, e2, e3);
+    if ((listed & 3u) != 0u && listed <= c.nbr_cap) list[(size_t)(listed >> 2) * b.n + i] = make_uint4(e0, e1, e2, e3);
     b.nbr_count[i] = listed;
 
     float density = 0.0f;
@@ -449,7 +464,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c, c
         const uint4* __restrict__ own = list + i;
         const uint32_t groups = (listed + 3u) >> 2;
         for (uint32_t gi = 0; gi < groups; gi++) {
-            const uint4 e = own[(size_t)gi * c.n];
+            const uint4 e = own[(size_t)gi * b.n];
             const float4 q0 = load_f4(xr, e.x), q1 = load_f4(xr, e.y), q2 = load_f4(xr, e.z),
                          q3 = load_f4(xr, e.w);
             const uint32_t t = 4u * gi;
@@ -524,17 +539,19 @@ __device__ __forceinline__ void force_pair(ForceAcc& f, const float4& p4, const 
 // over this substep's neighbour list (one walk, separate accumulators), then
 // WallAcceleration and UpdatePositions (slang:336-360), then UpdateSpatialHash
 // of the new position.  Particles whose list overflowed walk their cells.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimConsts c, uint32_t nblocks) {
+template <int BLOCK, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimConsts c0, SceneTable table,
+                                                            uint32_t nblocks) {
     const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + threadIdx.x;
-    if (i >= c.n) return;
+    const SimConsts c = scene_consts<BATCH>(c0, table, i);
+    if (i - c.base >= c.n) return;
     const float4 p4 = b.xs[i];
     const float4 v4 = b.vs[i];
     const float di = b.rho[i];
     const float pdi = p4.w;
     const float pi = eos(di, c);
-    const auto xr = make_rsrc(b.xs, c.n * 16u);
-    const auto vr = make_rsrc(b.vs, c.n * 16u);
+    const auto xr = make_rsrc(b.xs, b.n * 16u);
+    const auto vr = make_rsrc(b.vs, b.n * 16u);
     ForceAcc f{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 
     const uint32_t count = b.nbr_count[i];
@@ -546,7 +563,7 @@ __global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimCo
         const uint32_t groups = (count + 3u) >> 2;
         uint4 e = groups ? list[0] : make_uint4(0u, 0u, 0u, 0u);
         for (uint32_t g = 0; g < groups; g++) {
-            const uint4 en = list[(size_t)min(g + 1u, groups - 1u) * c.n];
+            const uint4 en = list[(size_t)min(g + 1u, groups - 1u) * b.n];
             const float4 q0 = load_f4(xr, e.x), w0 = load_f4(vr, e.x);
             const float4 q1 = load_f4(xr, e.y), w1 = load_f4(vr, e.y);
             const float4 q2 = load_f4(xr, e.z), w2 = load_f4(vr, e.z);
@@ -610,9 +627,9 @@ __global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimCo
     b.v[i] = make_float4(nvx, nvy, nvz, 0.0f);
     if (b.accel) b.accel[i] = make_float4(ax, ay, az, 0.0f);
     if (b.pid) b.pid[i] = b.pids[i];
-    b.keys[i] = fastmod(hash_cell(floor_to_cell(nxx / c.cell), floor_to_cell(nxy / c.cell),
-                                  floor_to_cell(nxz / c.cell)),
-                        c.fm);
+    b.keys[i] = c.base + fastmod(hash_cell(floor_to_cell(nxx / c.cell), floor_to_cell(nxy / c.cell),
+                                           floor_to_cell(nxz / c.cell)),
+                                 c.fm);
 }
 
 // ---------------------------------------------------------- volume maps
@@ -725,34 +742,48 @@ int launch_reorder_state(const StepBuffers& b, hipStream_t s) {
     return check_launch("reorder_state");
 }
 
-int launch_density(const StepBuffers& b, const SimConsts& c, const DevObject* objs, uint32_t n_obj, int block,
-                   int staged, hipStream_t s) {
+int launch_density(const StepBuffers& b, const SimConsts& c, const SceneTable* table, const DevObject* objs,
+                   int block, int staged, hipStream_t s) {
     if (b.n == 0) return 0;
     const uint32_t bs = block == 128 ? 128u : 256u;
     const uint32_t nb = particle_blocks(b.n, bs);
-    if (bs == 128) {
-        if (staged)
-            hipLaunchKernelGGL((k_density<128, true>), dim3(nb), dim3(128), 0, s, b, c, objs, n_obj, nb);
-        else
-            hipLaunchKernelGGL((k_density<128, false>), dim3(nb), dim3(128), 0, s, b, c, objs, n_obj, nb);
+    const SceneTable t = table ? *table : SceneTable{nullptr, nullptr};
+#define VFS_DENSITY(BS, ST, BA) \
+    hipLaunchKernelGGL((k_density<BS, ST, BA>), dim3(nb), dim3(BS), 0, s, b, c, t, objs, nb)
+    if (table) {
+        if (bs == 128) {
+            if (staged) VFS_DENSITY(128, true, true); else VFS_DENSITY(128, false, true);
+        } else {
+            if (staged) VFS_DENSITY(256, true, true); else VFS_DENSITY(256, false, true);
+        }
     } else {
-        if (staged)
-            hipLaunchKernelGGL((k_density<256, true>), dim3(nb), dim3(256), 0, s, b, c, objs, n_obj, nb);
-        else
-            hipLaunchKernelGGL((k_density<256, false>), dim3(nb), dim3(256), 0, s, b, c, objs, n_obj, nb);
+        if (bs == 128) {
+            if (staged) VFS_DENSITY(128, true, false); else VFS_DENSITY(128, false, false);
+        } else {
+            if (staged) VFS_DENSITY(256, true, false); else VFS_DENSITY(256, false, false);
+        }
     }
+#undef VFS_DENSITY
     return check_launch("density");
 }
 
-int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, int block, int staged, hipStream_t s) {
+int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, const SceneTable* table, int block,
+                            hipStream_t s) {
     if (b.n == 0) return 0;
-    (void)staged;  // the force pass reads neighbour lists; row staging only affects the density pass
     const uint32_t bs = block == 128 ? 128u : 256u;
     const uint32_t nb = particle_blocks(b.n, bs);
-    if (bs == 128)
-        hipLaunchKernelGGL((k_forces_integrate<128>), dim3(nb), dim3(128), 0, s, b, c, nb);
-    else
-        hipLaunchKernelGGL((k_forces_integrate<256>), dim3(nb), dim3(256), 0, s, b, c, nb);
+    const SceneTable t = table ? *table : SceneTable{nullptr, nullptr};
+    if (table) {
+        if (bs == 128)
+            hipLaunchKernelGGL((k_forces_integrate<128, true>), dim3(nb), dim3(128), 0, s, b, c, t, nb);
+        else
+            hipLaunchKernelGGL((k_forces_integrate<256, true>), dim3(nb), dim3(256), 0, s, b, c, t, nb);
+    } else {
+        if (bs == 128)
+            hipLaunchKernelGGL((k_forces_integrate<128, false>), dim3(nb), dim3(128), 0, s, b, c, t, nb);
+        else
+            hipLaunchKernelGGL((k_forces_integrate<256, false>), dim3(nb), dim3(256), 0, s, b, c, t, nb);
+    }
     return check_launch("forces_integrate");
 }
 
