@@ -158,6 +158,89 @@ def cpu_baseline(w, grids, steps: int) -> dict:
                       f"(FP32 oracle, OpenMP, {dt:.1f} s)"}
 
 
+def bench_batched(n_scenes: int, steps: int, looped_steps: int) -> dict:
+    """256 independent dam breaks of uneven sizes (log-uniform 2,048-65,536
+    particles, own boxes and 32^3 wall maps): one vfs_batch (autotuned launch
+    configuration, cell size included) against a loop over one vfs_sim per
+    scene, each replaying its own substep graph.  Throughput in
+    particle-substeps/s and scene-substeps/s."""
+    import ctypes as C
+
+    import vkfluidsim_amd as V
+    from vkfluidsim_amd import configs as CF
+
+    ws = CF.batched_dam_breaks(n_scenes=n_scenes)
+    grids = [[V.volume_map(kind, sp, res, w.h) for kind, sp, res in w.shapes] for w in ws]
+    params = [V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos) for w in ws]
+    total = sum(w.n for w in ws)
+
+    batch = V.Batch(params, grids)
+    for k, w in enumerate(ws):
+        batch.set_state(k, w.x, w.v)
+    batch.substeps(2)
+    batch.synchronize()
+    tuning, tuned_ms = batch.autotune(3, allow_cell_size=True)
+    batch.substeps(5)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.substeps(steps)
+    batch.synchronize()
+    tb = (time.perf_counter() - t0) / steps
+    finite = all(bool(np.isfinite(batch.get_state(k)[0]).all()) for k in range(n_scenes))
+    list_mean, list_max, list_over = batch.neighbour_stats()
+    stages = batch.profile(3)
+    batch.close()
+
+    sims = []
+    for k, w in enumerate(ws):
+        sim = V.Sim(params[k], grids[k])
+        sim.set_state(w.x, w.v)
+        sim.substeps(1)
+        sims.append(sim)
+    for sim in sims:
+        sim.synchronize()
+    # (a) every scene on its own stream: graphs of different scenes may overlap
+    t0 = time.perf_counter()
+    for _ in range(looped_steps):
+        for sim in sims:
+            sim.substeps(1)
+    for sim in sims:
+        sim.synchronize()
+    tl = (time.perf_counter() - t0) / looped_steps
+    # (b) one stream for the whole loop: scene after scene
+    stream = C.c_void_p()
+    V._lib.call("vfs_stream_create", C.byref(stream))
+    for sim in sims:
+        sim.substeps(1, stream)
+    V._lib.call("vfs_stream_synchronize", stream)
+    t0 = time.perf_counter()
+    for _ in range(looped_steps):
+        for sim in sims:
+            sim.substeps(1, stream)
+    V._lib.call("vfs_stream_synchronize", stream)
+    ts = (time.perf_counter() - t0) / looped_steps
+    V._lib.call("vfs_stream_destroy", stream)
+    for sim in sims:
+        sim.close()
+    return {
+        "workload": f"{n_scenes} dam breaks, {total} particles (log-uniform 2048-65536 per scene), 32^3 wall maps",
+        "batched": {"particle_steps_per_s": total / tb, "scene_steps_per_s": n_scenes / tb,
+                    "ms_per_substep": tb * 1e3, "substeps": steps,
+                    "tuning": list(tuning.as_tuple()), "tuning_keeps_reference_hash": tuning.cell_scale == 1.0,
+                    "stages_ms": {name: round(ms, 4) for name, ms, _ in stages}, "finite": finite,
+                    "neighbour_list": {"mean": round(list_mean, 3), "max": list_max, "overflowed": list_over}},
+        "looped": {"particle_steps_per_s": total / ts, "scene_steps_per_s": n_scenes / ts,
+                   "ms_per_substep": ts * 1e3, "substeps": looped_steps,
+                   "how": "one vfs_sim per scene, each replaying its substep graph, scene after scene on one stream"},
+        "looped_own_streams": {"particle_steps_per_s": total / tl, "scene_steps_per_s": n_scenes / tl,
+                               "ms_per_substep": tl * 1e3, "substeps": looped_steps,
+                               "how": "as looped, each scene on its own stream (graphs of different scenes "
+                                      "may overlap on the GPU)"},
+        "speedup": ts / tb,
+        "speedup_vs_own_streams": tl / tb,
+    }
+
+
 def summarize_stats(db: str) -> str:
     """Text table of a rocprofv3 --kernel-trace --stats database (top_kernels)."""
     import sqlite3
@@ -212,6 +295,10 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc child runs")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-scene batched vs looped measurement")
+    ap.add_argument("--batched-scenes", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=200)
+    ap.add_argument("--looped-steps", type=int, default=20)
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--summarize-stats", metavar="DB", help="print a rocprofv3 --stats database as text")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
@@ -309,6 +396,8 @@ def main() -> int:
         "event_sum_ms_per_step": sum(stage_ms.values()),
         "end_state": end_state,
     }
+    out["batched"] = None if args.no_batched else bench_batched(args.batched_scenes, args.batched_steps,
+                                                                  args.looped_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, grids, args.cpu_steps)
     else:

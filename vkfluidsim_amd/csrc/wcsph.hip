@@ -93,10 +93,10 @@ __device__ __forceinline__ SimConsts scene_consts(const SimConsts& c0, const Sce
     return t.consts[s];
 }
 
-// SampleGrid, slang:51-66, base cell clamped to [0, n-2].
-__device__ float sample_grid(const float* __restrict__ grid, const uint32_t* res, const float* uvw) {
+// SampleGrid, slang:51-66, base cell clamped to [0, n-2], in three parts: the
+// cell and its local coordinates, the eight corner values, the trilinear sum.
+__device__ __forceinline__ uint32_t grid_cell(const uint32_t* res, const float* uvw, float* chi) {
     uint32_t ci[3];
-    float chi[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const float x = ((float)res[a] - 1.0f) * uvw[a];
@@ -110,21 +110,36 @@ __device__ float sample_grid(const float* __restrict__ grid, const uint32_t* res
         ci[a] = cc;
         chi[a] = 2.0f * (x - (float)cc) - 1.0f;
     }
+    return ci[0] + res[0] * ci[1] + (res[0] * res[1]) * ci[2];
+}
+
+// corners in the order (0,0,0),(0,1,0),(0,0,1),(0,1,1),(1,0,0),(1,1,0),(1,0,1),(1,1,1)
+__device__ __forceinline__ void grid_corners(const float* __restrict__ grid, const uint32_t* res, uint32_t b0,
+                                             float* g) {
     const uint32_t sx = 1u, sy = res[0], sz = res[0] * res[1];
-    const uint32_t b0 = ci[0] + res[0] * ci[1] + sz * ci[2];
+    g[0] = grid[b0];
+    g[1] = grid[b0 + sy];
+    g[2] = grid[b0 + sz];
+    g[3] = grid[b0 + sy + sz];
+    g[4] = grid[b0 + sx];
+    g[5] = grid[b0 + sx + sy];
+    g[6] = grid[b0 + sx + sz];
+    g[7] = grid[b0 + sx + sy + sz];
+}
+
+__device__ __forceinline__ float grid_interp(const float* g, const float* chi) {
     const float mx = 1.0f + (-1.0f) * chi[0], px = 1.0f + chi[0];
     const float my = 1.0f + (-1.0f) * chi[1], py = 1.0f + chi[1];
     const float mz = 1.0f + (-1.0f) * chi[2], pz = 1.0f + chi[2];
-    // offsets (0,0,0),(0,1,0),(0,0,1),(0,1,1),(1,0,0),(1,1,0),(1,0,1),(1,1,1)
     float val = 0.0f;
-    val += grid[b0] * ((mx * my) * mz);
-    val += grid[b0 + sy] * ((mx * py) * mz);
-    val += grid[b0 + sz] * ((mx * my) * pz);
-    val += grid[b0 + sy + sz] * ((mx * py) * pz);
-    val += grid[b0 + sx] * ((px * my) * mz);
-    val += grid[b0 + sx + sy] * ((px * py) * mz);
-    val += grid[b0 + sx + sz] * ((px * my) * pz);
-    val += grid[b0 + sx + sy + sz] * ((px * py) * pz);
+    val += g[0] * ((mx * my) * mz);
+    val += g[1] * ((mx * py) * mz);
+    val += g[2] * ((mx * my) * pz);
+    val += g[3] * ((mx * py) * pz);
+    val += g[4] * ((px * my) * mz);
+    val += g[5] * ((px * py) * mz);
+    val += g[6] * ((px * my) * pz);
+    val += g[7] * ((px * py) * pz);
     return val / 8.0f;
 }
 
@@ -153,25 +168,34 @@ __device__ void boundary_volume(const DevObject* __restrict__ objs, uint32_t n_o
         for (int a = 0; a < 3; a++)
             outside |= (lp[a] < o.box_pos[a]) || (lp[a] > o.box_pos[a] + o.box_size[a]);
         if (outside) continue;
-        float uvw[3];
+        float uvw[3], chi[3], gs[8], g8[8];
         to_uvw(o, lp, uvw);
-        const float dist = sample_grid(o.sdf, o.res, uvw);
+        const uint32_t cell = grid_cell(o.res, uvw, chi);
+        grid_corners(o.sdf, o.res, cell, gs);
+        const float dist = grid_interp(gs, chi);
         if (!(dist >= 0.0f && dist < c.h)) continue;
-        const float volume = sample_grid(o.vm, o.res, uvw);
+        grid_corners(o.vm, o.res, cell, g8);
+        const float volume = grid_interp(g8, chi);
         if (!(volume > 0.0f)) continue;
-        // GradientVector, slang:68-85
+        // GradientVector, slang:68-85; samples in the particle's grid cell
+        // reuse its SDF corners
         const float eps = 0.1f * c.h;
+        auto sdf_at = [&](const float* q) {
+            float u[3], ch[3];
+            to_uvw(o, q, u);
+            const uint32_t cq = grid_cell(o.res, u, ch);
+            if (cq == cell) return grid_interp(gs, ch);
+            grid_corners(o.sdf, o.res, cq, g8);
+            return grid_interp(g8, ch);
+        };
         float nrm[3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             float xp[3] = {lp[0], lp[1], lp[2]}, xm[3] = {lp[0], lp[1], lp[2]};
             xp[a] = lp[a] + eps;
             xm[a] = lp[a] - eps;
-            float u[3];
-            to_uvw(o, xp, u);
-            const float v1 = sample_grid(o.sdf, o.res, u);
-            to_uvw(o, xm, u);
-            const float v0 = sample_grid(o.sdf, o.res, u);
+            const float v1 = sdf_at(xp);
+            const float v0 = sdf_at(xm);
             nrm[a] = (v1 - v0) * (1.0f / (2.0f * eps));
         }
         const float modn = sqrtf(dot3(nrm[0], nrm[1], nrm[2], nrm[0], nrm[1], nrm[2]));
