@@ -241,6 +241,64 @@ def bench_batched(n_scenes: int, steps: int, looped_steps: int) -> dict:
     }
 
 
+def bench_adjoint(steps: int, every: int, repeats: int = 3) -> dict:
+    """Differentiable dam break: 65,536 particles (64 x 32 x 32 lattice, spacing
+    0.02, the 1.6 x 0.8 x 0.8 walled box), `steps` substeps, loss = mean squared
+    difference of the density on a 32^3 grid against the field of a run with
+    initial velocities perturbed by N(0, 0.01); gradients with respect to the
+    initial velocities (and positions) and the viscosity, checkpoints every
+    `every` substeps.  Reports forward + adjoint time over plain forward time."""
+    import vkfluidsim_amd as V
+    from vkfluidsim_amd import configs as CF
+
+    w = CF.dam_break(side=(64, 32, 32), seed=CF.SEED_BASE + 4, steps=steps)
+    grids = [V.volume_map(kind, sp, res, w.h) for kind, sp, res in w.shapes]
+    p = V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos)
+    sim = V.Sim(p, grids)
+    dims = (32, 32, 32)
+    spacing = tuple(b / (d - 1) for b, d in zip(w.box_size, dims))
+    adj = V.Adjoint(sim, steps, every, (0.0, 0.0, 0.0), spacing, dims)
+    v_t = (0.01 * CF.normal(CF.SEED_BASE + 4, 3 * w.n)).astype(np.float32).reshape(w.n, 3)
+    _, target = adj.forward(w.x, v_t)
+
+    def plain():
+        sim.set_state(w.x, w.v)
+        sim.substeps(1)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.substeps(steps)
+        sim.synchronize()
+        return time.perf_counter() - t0
+
+    def fwd_adj():
+        t0 = time.perf_counter()
+        loss, _ = adj.forward(w.x, w.v, target)
+        t1 = time.perf_counter()
+        gx, gv, gnu = adj.backward()
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, loss, gv, gnu
+
+    plain()
+    fwd_adj()
+    t_plain = min(plain() for _ in range(repeats))
+    runs = [fwd_adj() for _ in range(repeats)]
+    t_f = min(r[0] for r in runs)
+    t_b = min(r[1] for r in runs)
+    loss, gv, gnu = runs[-1][2], runs[-1][3], runs[-1][4]
+    adj.close()
+    sim.close()
+    return {
+        "workload": f"dam break {w.n} particles, {steps} substeps, checkpoints every {every}, 32^3 density grid",
+        "forward_ms": t_plain * 1e3,
+        "adjoint_forward_ms": t_f * 1e3,
+        "adjoint_backward_ms": t_b * 1e3,
+        "fwd_plus_adj_over_fwd": (t_f + t_b) / t_plain,
+        "loss": loss,
+        "grad_v0_norm": float(np.linalg.norm(gv)),
+        "grad_viscosity": gnu,
+    }
+
+
 def summarize_stats(db: str) -> str:
     """Text table of a rocprofv3 --kernel-trace --stats database (top_kernels)."""
     import sqlite3
@@ -299,6 +357,9 @@ def main() -> int:
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--looped-steps", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--summarize-stats", metavar="DB", help="print a rocprofv3 --stats database as text")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
@@ -398,6 +459,7 @@ def main() -> int:
     }
     out["batched"] = None if args.no_batched else bench_batched(args.batched_scenes, args.batched_steps,
                                                                   args.looped_steps)
+    out["adjoint"] = None if args.no_adjoint else bench_adjoint(args.adjoint_steps, args.adjoint_every)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, grids, args.cpu_steps)
     else:

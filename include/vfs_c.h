@@ -227,6 +227,39 @@ int vfs_batch_profile_stages(vfs_batch *batch, uint32_t n, float *ms, uint32_t *
                              const char **names, uint32_t max_stages, uint32_t *n_stages);
 int vfs_batch_neighbour_stats(vfs_batch *batch, double *mean, uint32_t *max_count, uint32_t *overflowed);
 
+/* ---- differentiable substeps ------------------------------------------- */
+/* No counterpart in the reference (it has no differentiation).  A hand-written
+ * adjoint of density, pressure, viscosity, boundary volume and integration,
+ * with the neighbour lists of each substep frozen.  The loss is the mean
+ * squared difference between the density field sampled on a grid,
+ * rho(g) = sum_j m W(|g - x_j|) over the particles within h of g, and a
+ * target field.  The forward pass keeps (x, v) every checkpoint_every
+ * substeps; the backward pass restores each checkpoint, recomputes its
+ * substeps (hash, sort, lists) and reverses them.  Single-scene simulations
+ * with the scene's own cell size only; every list must fit
+ * VFS_NEIGHBOUR_CAPACITY. */
+typedef struct vfs_density_grid {
+    float origin[3];   /* position of grid point (0, 0, 0) */
+    float spacing[3];
+    uint32_t dims[3];  /* point (i, j, k) is origin + (i, j, k) * spacing, index i + nx (j + ny k) */
+} vfs_density_grid;
+
+typedef struct vfs_adjoint vfs_adjoint;
+int vfs_adjoint_create(vfs_sim *sim, uint32_t n_substeps, uint32_t checkpoint_every,
+                       const vfs_density_grid *grid, vfs_adjoint **out);
+int vfs_adjoint_destroy(vfs_adjoint *adj);
+/* Sets the simulation's state from device AoS float3 arrays (n_particles x 3),
+ * runs n_substeps substeps and returns the loss.  target_dev: grid values on
+ * the device (NULL: zero target); density_dev: optional device output of the
+ * sampled density field. */
+int vfs_adjoint_forward(vfs_adjoint *adj, const float *x0_dev, const float *v0_dev, const float *target_dev,
+                        float *density_dev, double *loss);
+/* Gradients of the last forward's loss with respect to the initial positions
+ * and velocities (device AoS float3 outputs, either may be NULL) and to the
+ * viscosity parameter (viscosity_strenght).  The simulation is left at the
+ * forward's final positions and velocities. */
+int vfs_adjoint_backward(vfs_adjoint *adj, float *grad_x0_dev, float *grad_v0_dev, double *grad_viscosity);
+
 /* ---- compute layer drop-ins (reference src/compute headers) ------------ */
 
 /* SpatialHash (src/compute/spatial_hash.h:9-34): Init(n, cell_size), Run(positions),

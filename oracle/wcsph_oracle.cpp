@@ -31,6 +31,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -356,6 +357,17 @@ struct Sim {
     std::vector<uint32_t> tpid;
     int threads = 1;
 
+    // Per-substep record for the host adjoint: the sorted input state, the
+    // density-loop neighbours of every particle (self and repeats included, in
+    // loop order), densities, boundary terms and the permutation.
+    struct TapeStep {
+        std::vector<uint32_t> perm;
+        std::vector<R> x, v, rho, psi, bgrad;
+        std::vector<std::vector<uint32_t>> nbr;
+    };
+    std::vector<TapeStep>* tape = nullptr;
+    std::vector<std::vector<uint32_t>> lists;
+
     void hash_and_sort() {
         // UpdateSpatialHash (update_spatial_hash.slang:18-28)
 #pragma omp parallel for schedule(static) num_threads(threads)
@@ -404,6 +416,7 @@ struct Sim {
         for (uint32_t i = 0; i < n; i++)
             for (int c = 0; c < 3; c++) a[3 * i + c] = m.g[c];
         hash_and_sort();
+        if (tape && lists.size() != n) lists.assign(n, {});
         // CalculateBoundaryVolume
 #pragma omp parallel for schedule(static) num_threads(threads)
         for (int64_t i = 0; i < (int64_t)n; i++) boundary_volume<R>(m, &x[3 * i], psi[i], &bgrad[3 * i]);
@@ -428,10 +441,23 @@ struct Sim {
                     if (r2 > m.h2) continue;
                     const R r = std::sqrt(r2);
                     density += m.mass * m.W(r);
+                    if (tape) lists[i].push_back(j);
                 }
             }
             density += m.rho0 * psi[i];
             rho[i] = density;
+        }
+        if (tape) {
+            TapeStep t;
+            t.perm = perm;
+            t.x = x;
+            t.v = v;
+            t.rho = rho;
+            t.psi = psi;
+            t.bgrad = bgrad;
+            t.nbr.swap(lists);
+            tape->push_back(std::move(t));
+            lists.assign(n, {});
         }
         // CalculatePressureAccel (slang:205-271)
 #pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
@@ -581,6 +607,317 @@ struct Sim {
         }
     }
 };
+
+// ------------------------------------------------------ host adjoint (FP64)
+// Reverse pass of the substep over the recorded neighbour lists, and the
+// density-grid loss: the restatement of the GPU adjoint, in double, used to
+// check it (and checked itself against finite differences of the FP64 forward).
+struct GridC {
+    float origin[3], spacing[3];
+    uint32_t dims[3];
+};
+
+template <class R>
+static R kernel_ddw(const Model<R>& m, R r) {
+    const R q = r * m.inv_h;
+    if ((r > R(1e-9f)) && (q <= R(1))) {
+        if (q > R(0.5)) return ((R(2) * m.grad_sigma) * (R(1) - q)) * m.inv_h;
+        return (m.grad_sigma * (R(6) * q - R(2))) * m.inv_h;
+    }
+    return R(0);
+}
+
+template <class R>
+static R eos_dp(const Model<R>& m, R rho) {
+    if (!(rho > m.rho0)) return R(0);
+    return m.B * m.gamma * std::pow(rho / m.rho0, m.gamma - R(1)) / m.rho0;
+}
+
+template <class R>
+static void grid_point(const GridC& g, uint32_t ix, uint32_t iy, uint32_t iz, R* p) {
+    // the same float products as the device
+    p[0] = (R)(g.origin[0] + (float)ix * g.spacing[0]);
+    p[1] = (R)(g.origin[1] + (float)iy * g.spacing[1]);
+    p[2] = (R)(g.origin[2] + (float)iz * g.spacing[2]);
+}
+
+template <class R>
+static void grid_range(const GridC& g, const R* p, R h, int* lo, int* hi) {
+    for (int a = 0; a < 3; a++) {
+        lo[a] = (int)std::max(std::floor((double)(p[a] - h - (R)g.origin[a]) / g.spacing[a]) - 1.0, 0.0);
+        hi[a] = (int)std::min(std::ceil((double)(p[a] + h - (R)g.origin[a]) / g.spacing[a]) + 1.0,
+                              (double)g.dims[a] - 1.0);
+    }
+}
+
+// rho(g) = sum_j m W(|g - x_j|) over particles within h.
+template <class R>
+static void grid_density(const Model<R>& m, const std::vector<R>& x, uint32_t n, const GridC& g,
+                         std::vector<R>& out) {
+    out.assign((size_t)g.dims[0] * g.dims[1] * g.dims[2], R(0));
+    for (uint32_t j = 0; j < n; j++) {
+        int lo[3], hi[3];
+        grid_range<R>(g, &x[3 * j], m.h, lo, hi);
+        for (int iz = lo[2]; iz <= hi[2]; iz++)
+            for (int iy = lo[1]; iy <= hi[1]; iy++)
+                for (int ix = lo[0]; ix <= hi[0]; ix++) {
+                    R gp[3], d[3];
+                    grid_point<R>(g, ix, iy, iz, gp);
+                    for (int a = 0; a < 3; a++) d[a] = x[3 * j + a] - gp[a];
+                    const R r2 = dot3<R>(d, d);
+                    if (r2 > m.h2) continue;
+                    out[(size_t)ix + g.dims[0] * ((size_t)iy + (size_t)g.dims[1] * iz)] += m.mass * m.W(std::sqrt(r2));
+                }
+    }
+}
+
+template <class R>
+static void stencil_keys(const Model<R>& m, const R* p, uint32_t n, uint32_t* k) {
+    int32_t c[3];
+    get_cell<R>(p, m.cell, c);
+    for (int o = 0; o < 27; o++)
+        k[o] = hash_cell(c[0] + kOffsets3D[o][0], c[1] + kOffsets3D[o][1], c[2] + kOffsets3D[o][2]) % n;
+}
+
+template <class R>
+static R pair_weight(const Model<R>& m, const R* xi, const R* xj, uint32_t n) {
+    uint32_t ki[27], kj[27];
+    stencil_keys<R>(m, xi, n, ki);
+    stencil_keys<R>(m, xj, n, kj);
+    // own keys are the stencil's centre (offset 13)
+    uint32_t mij = 0, mji = 0;
+    for (int o = 0; o < 27; o++) {
+        mij += ki[o] == kj[13] ? 1u : 0u;
+        mji += kj[o] == ki[13] ? 1u : 0u;
+    }
+    return mij ? (R)mji / (R)mij : R(0);
+}
+
+// value and d/duvw of SampleGrid with its base cell held fixed
+template <class R>
+static R sample_grid_grad(const float* grid, const uint32_t* n, const R* uvw, R* grad) {
+    uint32_t c[3];
+    R chi[3];
+    for (int a = 0; a < 3; a++) {
+        const R x = (R(n[a]) - R(1)) * uvw[a];
+        const R f = std::floor(x);
+        const uint32_t hi = n[a] >= 2 ? n[a] - 2 : 0;
+        c[a] = f >= R(0) ? (f <= R(hi) ? (uint32_t)f : hi) : 0;
+        chi[a] = R(2) * (x - R(c[a])) - R(1);
+    }
+    static const uint32_t off[8][3] = {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1},
+                                       {1, 0, 0}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}};
+    R val = R(0);
+    grad[0] = grad[1] = grad[2] = R(0);
+    for (int i = 0; i < 8; i++) {
+        R f[3], s[3];
+        for (int a = 0; a < 3; a++) {
+            s[a] = off[i][a] ? R(1) : R(-1);
+            f[a] = R(1) + s[a] * chi[a];
+        }
+        const uint32_t idx = (c[0] + off[i][0]) + n[0] * (c[1] + off[i][1]) + n[0] * n[1] * (c[2] + off[i][2]);
+        const R gv = R(grid[idx]);
+        val += gv * ((f[0] * f[1]) * f[2]);
+        grad[0] += gv * s[0] * f[1] * f[2];
+        grad[1] += gv * f[0] * s[1] * f[2];
+        grad[2] += gv * f[0] * f[1] * s[2];
+    }
+    for (int a = 0; a < 3; a++) grad[a] *= R(2) * (R(n[a]) - R(1)) / R(8);
+    return val / R(8);
+}
+
+template <class R>
+static void boundary_adjoint(const Model<R>& m, const R* pos, R psi_bar, const R* gb, R* gpos) {
+    for (const auto& o : m.objs) {
+        R lp[3];
+        mat_vec<R>(o.T, pos, R(1), lp);
+        bool outside = false;
+        for (int a = 0; a < 3; a++)
+            if (lp[a] < o.box_pos[a] || lp[a] > o.box_pos[a] + o.box_size[a]) outside = true;
+        if (outside) continue;
+        R uvw[3], gS[3], gV[3];
+        to_uvw<R>(o.box_pos, o.box_size, lp, uvw);
+        const R dist = sample_grid_grad<R>(o.sdf.data(), o.res, uvw, gS);
+        if (!(dist >= R(0) && dist < m.h)) continue;
+        const R vol = sample_grid_grad<R>(o.vm.data(), o.res, uvw, gV);
+        if (!(vol > R(0))) continue;
+        for (int a = 0; a < 3; a++) {
+            gS[a] /= o.box_size[a];
+            gV[a] /= o.box_size[a];
+        }
+        const R eps = R(0.1f) * m.h;
+        R nrm[3], Jn[3][3];
+        for (int a = 0; a < 3; a++) {
+            R v[2], g[2][3];
+            for (int sg = 0; sg < 2; sg++) {
+                R q[3] = {lp[0], lp[1], lp[2]};
+                q[a] = sg == 0 ? lp[a] + eps : lp[a] - eps;
+                R u[3];
+                to_uvw<R>(o.box_pos, o.box_size, q, u);
+                v[sg] = sample_grid_grad<R>(o.sdf.data(), o.res, u, g[sg]);
+            }
+            nrm[a] = (v[0] - v[1]) / (R(2) * eps);
+            for (int b = 0; b < 3; b++) Jn[a][b] = (g[0][b] - g[1][b]) / o.box_size[b] / (R(2) * eps);
+        }
+        const R modn = std::sqrt(dot3<R>(nrm, nrm));
+        if (!(modn > R(1e-9f))) continue;
+        R Rn[3], RJ[3][3], nJ[3];
+        for (int r = 0; r < 3; r++) {
+            Rn[r] = o.Rm[r] * nrm[0] + o.Rm[4 + r] * nrm[1] + o.Rm[8 + r] * nrm[2] + o.Rm[12 + r];
+            for (int b = 0; b < 3; b++) RJ[r][b] = o.Rm[r] * Jn[0][b] + o.Rm[4 + r] * Jn[1][b] + o.Rm[8 + r] * Jn[2][b];
+        }
+        for (int b = 0; b < 3; b++) nJ[b] = nrm[0] * Jn[0][b] + nrm[1] * Jn[1][b] + nrm[2] * Jn[2][b];
+        R rn[3], Jr[3][3];
+        for (int r = 0; r < 3; r++) {
+            rn[r] = Rn[r] / modn;
+            for (int b = 0; b < 3; b++) Jr[r][b] = RJ[r][b] / modn - Rn[r] * nJ[b] / (modn * modn * modn);
+        }
+        const R dd = dist + R(0.125f) * m.h;
+        const R d = std::fmax(dd, R(0.5f) * m.h);
+        R xij[3], Jx[3][3];
+        for (int r = 0; r < 3; r++) {
+            xij[r] = d * rn[r];
+            for (int b = 0; b < 3; b++) Jx[r][b] = (dd > R(0.5f) * m.h ? rn[r] * gS[b] : R(0)) + d * Jr[r][b];
+        }
+        const R xm = std::sqrt(dot3<R>(xij, xij));
+        if (!(xm > R(0))) continue;
+        R xh[3], dxm[3], perp[3];
+        for (int r = 0; r < 3; r++) xh[r] = xij[r] / xm;
+        for (int b = 0; b < 3; b++) dxm[b] = xh[0] * Jx[0][b] + xh[1] * Jx[1][b] + xh[2] * Jx[2][b];
+        const R W = m.W(xm), G = m.dW(xm), Gp = kernel_ddw<R>(m, xm), Wp = m.h * G;
+        const R xg = xh[0] * gb[0] + xh[1] * gb[1] + xh[2] * gb[2];
+        for (int r = 0; r < 3; r++) perp[r] = (gb[r] - xg * xh[r]) / xm;
+        R glp[3];
+        for (int b = 0; b < 3; b++) {
+            const R jp = Jx[0][b] * perp[0] + Jx[1][b] * perp[1] + Jx[2][b] * perp[2];
+            glp[b] = psi_bar * (W * gV[b] + vol * Wp * dxm[b]) + xg * (G * gV[b] + vol * Gp * dxm[b]) + vol * G * jp;
+        }
+        for (int b = 0; b < 3; b++) gpos[b] += o.T[b * 4 + 0] * glp[0] + o.T[b * 4 + 1] * glp[1] + o.T[b * 4 + 2] * glp[2];
+    }
+}
+
+// One substep backwards: gx/gv hold the adjoints of the substep's outputs
+// (sorted order) on entry and of its inputs (input order) on return.
+template <class R>
+static void adjoint_substep(const Model<R>& m, const typename Sim<R>::TapeStep& t, uint32_t n, std::vector<R>& gx,
+                            std::vector<R>& gv, R& nu_bar) {
+    std::vector<R> ab(3 * (size_t)n), ax(3 * (size_t)n), av(3 * (size_t)n), rbar(n), gpsi(3 * (size_t)n);
+    std::vector<uint8_t> dup(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const R* p = &t.x[3 * i];
+        R diff[3];
+        bool wall = false;
+        for (int k = 0; k < 3; k++) {
+            diff[k] = std::fmin(p[k] - m.bb_lo[k], R(0)) - std::fmin(m.bb_hi[k] - p[k], R(0));
+            if (std::fabs(diff[k]) > R(1e-9f)) wall = true;
+        }
+        for (int k = 0; k < 3; k++) {
+            const R vt = gv[3 * i + k] + m.dt * gx[3 * i + k];
+            ab[3 * i + k] = m.dt * vt;
+            av[3 * i + k] = vt;
+            ax[3 * i + k] = gx[3 * i + k];
+            if (wall) {
+                const R D = (p[k] < m.bb_lo[k] ? R(1) : R(0)) + (p[k] > m.bb_hi[k] ? R(1) : R(0));
+                ax[3 * i + k] += -m.kwall * D * ab[3 * i + k];
+            }
+        }
+        uint32_t k27[27];
+        stencil_keys<R>(m, p, n, k27);
+        bool d = false;
+        for (int u = 1; u < 27 && !d; u++)
+            for (int w = 0; w < u; w++) d |= k27[u] == k27[w];
+        dup[i] = d ? 1 : 0;
+    }
+    const R nu10 = (R(2) * (R(3) + R(2))) * m.nu;
+    auto P = [&](uint32_t j) {
+        const R rj = t.rho[j];
+        return rj > R(0) ? m.eos(rj) / (rj * rj) : R(0);
+    };
+    auto M = [&](uint32_t j) { return m.mass / std::fmax(t.rho[j], R(1e-6f)); };
+    for (uint32_t i = 0; i < n; i++) {
+        const R* xi = &t.x[3 * i];
+        const R* vi = &t.v[3 * i];
+        const R Pi = P(i), Mi = M(i);
+        const R* ai = &ab[3 * i];
+        R Pbar = 0, Mbar = 0, gxl[3] = {0, 0, 0}, gvl[3] = {0, 0, 0}, fv[3] = {0, 0, 0};
+        for (uint32_t j : t.nbr[i]) {
+            if (j == i) continue;
+            const R* xj = &t.x[3 * j];
+            const R* vj = &t.v[3 * j];
+            R xij[3], vij[3];
+            for (int k = 0; k < 3; k++) {
+                xij[k] = xi[k] - xj[k];
+                vij[k] = vi[k] - vj[k];
+            }
+            const R r2 = dot3<R>(xij, xij), r = std::sqrt(r2);
+            if (!(r > R(1e-9f))) continue;
+            R nh[3], F[3];
+            const R G = m.dW(r), Gp = kernel_ddw<R>(m, r);
+            for (int k = 0; k < 3; k++) {
+                nh[k] = xij[k] / r;
+                F[k] = nh[k] * G;
+            }
+            const R sig = dot3<R>(vij, xij);
+            const R inv_den = R(1) / (r2 + R(0.01f) * m.h2);
+            const R Pj = P(j), Mj = M(j);
+            const R* aj = &ab[3 * j];
+            const R w = (dup[i] || dup[j]) ? pair_weight<R>(m, xi, xj, n) : R(1);
+            for (int k = 0; k < 3; k++) fv[k] += Mj * sig * inv_den * F[k];
+            const R aF = ai[0] * F[0] + ai[1] * F[1] + ai[2] * F[2];
+            const R bF = -(aj[0] * F[0] + aj[1] * F[1] + aj[2] * F[2]);
+            Pbar += -m.mass * aF + w * (-m.mass * bF);
+            Mbar += w * (nu10 * sig * inv_den * bF);
+            R Fb[3];
+            for (int k = 0; k < 3; k++)
+                Fb[k] = (-m.mass * (Pi + Pj) + nu10 * Mj * sig * inv_den) * ai[k] -
+                        w * (-m.mass * (Pj + Pi) + nu10 * Mi * sig * inv_den) * aj[k];
+            const R sb = nu10 * Mj * inv_den * aF + w * nu10 * Mi * inv_den * bF;
+            const R db = -nu10 * Mj * sig * inv_den * inv_den * aF - w * nu10 * Mi * sig * inv_den * inv_den * bF;
+            const R Fn = Fb[0] * nh[0] + Fb[1] * nh[1] + Fb[2] * nh[2];
+            for (int k = 0; k < 3; k++) {
+                gxl[k] += (G / r) * Fb[k] + Fn * (Gp - G / r) * nh[k] + sb * vij[k] + R(2) * db * xij[k];
+                gvl[k] += sb * xij[k];
+            }
+        }
+        nu_bar += R(10) * (ai[0] * fv[0] + ai[1] * fv[1] + ai[2] * fv[2]);
+        const R rhoi = t.rho[i];
+        const R p = m.eos(rhoi);
+        const R pb = rhoi > R(0) ? p / (m.rho0 * m.rho0) : R(0);
+        const R ag = ai[0] * t.bgrad[3 * i] + ai[1] * t.bgrad[3 * i + 1] + ai[2] * t.bgrad[3 * i + 2];
+        Pbar += -m.rho0 * ag;
+        for (int k = 0; k < 3; k++) gpsi[3 * i + k] = -(Pi + pb) * m.rho0 * ai[k];
+        const R dp = eos_dp<R>(m, rhoi);
+        R rb = rhoi > R(0) ? -m.rho0 * ag * dp / (m.rho0 * m.rho0) : R(0);
+        if (rhoi > R(0)) rb += Pbar * (dp / (rhoi * rhoi) - R(2) * p / (rhoi * rhoi * rhoi));
+        if (rhoi > R(1e-6f)) rb += Mbar * (-m.mass / (rhoi * rhoi));
+        rbar[i] = rb;
+        for (int k = 0; k < 3; k++) {
+            ax[3 * i + k] += gxl[k];
+            av[3 * i + k] += gvl[k];
+        }
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        const R* xi = &t.x[3 * i];
+        R g[3] = {ax[3 * i], ax[3 * i + 1], ax[3 * i + 2]};
+        for (uint32_t j : t.nbr[i]) {
+            if (j == i) continue;
+            const R* xj = &t.x[3 * j];
+            R xij[3];
+            for (int k = 0; k < 3; k++) xij[k] = xi[k] - xj[k];
+            const R r = std::sqrt(dot3<R>(xij, xij));
+            if (!(r > R(1e-9f))) continue;
+            const R w = (dup[i] || dup[j]) ? pair_weight<R>(m, xi, xj, n) : R(1);
+            const R s = m.mass * (m.h * m.dW(r)) * (rbar[i] + w * rbar[j]) / r;
+            for (int k = 0; k < 3; k++) g[k] += s * xij[k];
+        }
+        if (!m.objs.empty()) boundary_adjoint<R>(m, xi, m.rho0 * rbar[i], &gpsi[3 * i], g);
+        const uint32_t o = t.perm[i];
+        for (int k = 0; k < 3; k++) {
+            gx[3 * o + k] = g[k];
+            gv[3 * o + k] = av[3 * i + k];
+        }
+    }
+}
 
 // ------------------------------------------------------- volume map (FP64)
 // Gauss-Legendre nodes/weights on [-1,1] by Newton iteration on P_n.
@@ -781,6 +1118,80 @@ void orc_sim_set_state(void* h, const float* x, const float* v, uint32_t offset,
         set(s->s64);
     else
         set(s->s32);
+}
+
+// State in double (rounded to float for the FP32 instance): finite
+// differences of the FP64 instance need perturbations below float resolution.
+void orc_sim_set_state_f64(void* h, const double* x, const double* v, uint32_t offset, uint32_t count) {
+    auto* s = (OrcSim*)h;
+    auto set = [&](auto& sim) {
+        using R = typename std::decay<decltype(sim.x[0])>::type;
+        for (size_t i = 0; i < 3 * (size_t)count; i++) {
+            sim.x[3 * (size_t)offset + i] = (R)x[i];
+            sim.v[3 * (size_t)offset + i] = (R)v[i];
+        }
+        for (uint32_t i = 0; i < count; i++) sim.rho[offset + i] = 0;
+    };
+    if (s->fp64)
+        set(s->s64);
+    else
+        set(s->s32);
+}
+
+// FP64 host adjoint: from the simulation's current state, k substeps with the
+// density-grid loss against target; returns the loss and writes the gradients
+// with respect to the state at the start (positions, velocities, AoS) and to
+// the viscosity.  The simulation is left at the final state.  FP64 instance only.
+int orc_adjoint_run(void* h, uint32_t k, const orc::GridC* grid, const double* target, double* loss,
+                    double* grad_x, double* grad_v, double* grad_nu, double* density) {
+    auto* o = (OrcSim*)h;
+    if (!o->fp64 || !grid) return -1;
+    auto& sim = o->s64;
+    const uint32_t n = sim.n;
+    std::vector<orc::Sim<double>::TapeStep> tape;
+    sim.tape = &tape;
+    for (uint32_t s = 0; s < k; s++) sim.substep();
+    sim.tape = nullptr;
+    sim.lists.clear();
+    std::vector<double> rg;
+    orc::grid_density<double>(sim.m, sim.x, n, *grid, rg);
+    const size_t G = rg.size();
+    double L = 0.0;
+    for (size_t g = 0; g < G; g++) {
+        const double d = rg[g] - (target ? target[g] : 0.0);
+        L += d * d;
+    }
+    if (loss) *loss = L / (double)G;
+    if (density)
+        for (size_t g = 0; g < G; g++) density[g] = rg[g];
+    std::vector<double> gx(3 * (size_t)n, 0.0), gv(3 * (size_t)n, 0.0);
+    for (uint32_t j = 0; j < n; j++) {
+        int lo[3], hi[3];
+        orc::grid_range<double>(*grid, &sim.x[3 * j], sim.m.h, lo, hi);
+        for (int iz = lo[2]; iz <= hi[2]; iz++)
+            for (int iy = lo[1]; iy <= hi[1]; iy++)
+                for (int ix = lo[0]; ix <= hi[0]; ix++) {
+                    double gp[3], d[3];
+                    orc::grid_point<double>(*grid, ix, iy, iz, gp);
+                    for (int a = 0; a < 3; a++) d[a] = sim.x[3 * j + a] - gp[a];
+                    const double r2 = orc::dot3<double>(d, d);
+                    if (r2 > sim.m.h2) continue;
+                    const double r = std::sqrt(r2);
+                    if (!(r > 1e-9)) continue;
+                    const size_t idx = (size_t)ix + grid->dims[0] * ((size_t)iy + (size_t)grid->dims[1] * iz);
+                    const double cg = 2.0 * (rg[idx] - (target ? target[idx] : 0.0)) / (double)G;
+                    const double s2 = cg * sim.m.mass * (sim.m.h * sim.m.dW(r)) / r;
+                    for (int a = 0; a < 3; a++) gx[3 * j + a] += s2 * d[a];
+                }
+    }
+    double nu_bar = 0.0;
+    for (int s = (int)k - 1; s >= 0; s--) orc::adjoint_substep<double>(sim.m, tape[s], n, gx, gv, nu_bar);
+    for (size_t i = 0; i < 3 * (size_t)n; i++) {
+        if (grad_x) grad_x[i] = gx[i];
+        if (grad_v) grad_v[i] = gv[i];
+    }
+    if (grad_nu) *grad_nu = nu_bar;
+    return 0;
 }
 
 void orc_sim_substeps(void* h, uint32_t k) {

@@ -36,6 +36,8 @@ def _load_oracle() -> C.CDLL:
         "orc_sim_destroy": (None, [vp]),
         "orc_sim_set_threads": (None, [vp, C.c_int]),
         "orc_sim_set_state": (None, [vp, fp, fp, u32, u32]),
+        "orc_sim_set_state_f64": (None, [vp, dp, dp, u32, u32]),
+        "orc_adjoint_run": (C.c_int, [vp, u32, vp, dp, dp, dp, dp, dp, dp]),
         "orc_sim_substeps": (None, [vp, u32]),
         "orc_sim_get_state": (None, [vp, dp, dp, dp, up, dp]),
         "orc_sim_get_state_f32": (None, [vp, fp, fp, fp]),
@@ -95,6 +97,11 @@ class OracleSim:
         x = np.ascontiguousarray(x, np.float32).reshape(-1)
         v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, np.float32).reshape(-1)
         ORC.orc_sim_set_state(self.h, fp(x), fp(v), 0, x.size // 3)
+
+    def set_state_f64(self, x, v=None):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
+        v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, np.float64).reshape(-1)
+        ORC.orc_sim_set_state_f64(self.h, dp(x), dp(v), 0, x.size // 3)
 
     def substeps(self, k: int):
         ORC.orc_sim_substeps(self.h, int(k))
@@ -156,6 +163,32 @@ def hash_cells(c):
     lo, hi = u % 50, u // 50
     h = lo[:, 0] + 50 * (lo[:, 1] + 50 * lo[:, 2]) + hi[:, 0] * 15823 + hi[:, 1] * 9737333 + hi[:, 2] * 440817757
     return h & 0xFFFFFFFF
+
+
+class GridC(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("spacing", C.c_float * 3), ("dims", C.c_uint32 * 3)]
+
+
+def oracle_adjoint(params, grids, x0, v0, steps, origin, spacing, dims, target=None, nthreads=None):
+    """FP64 host adjoint: (loss, density field, d loss/d x0, d loss/d v0, d loss/d viscosity)."""
+    o = OracleSim(params, grids, fp64=True, nthreads=nthreads)
+    o.set_state_f64(x0, v0)
+    g = GridC()
+    g.origin[:] = [float(v) for v in origin]
+    g.spacing[:] = [float(v) for v in spacing]
+    g.dims[:] = [int(v) for v in dims]
+    n = int(params.n_particles)
+    G = int(np.prod([int(v) for v in dims]))
+    t = None if target is None else np.ascontiguousarray(target, np.float64)
+    loss = np.zeros(1)
+    rho = np.empty(G)
+    gx = np.empty(3 * n)
+    gv = np.empty(3 * n)
+    gnu = np.zeros(1)
+    rc = ORC.orc_adjoint_run(o.h, int(steps), C.addressof(g), dp(t) if t is not None else None, dp(loss), dp(gx),
+                             dp(gv), dp(gnu), dp(rho))
+    assert rc == 0
+    return float(loss[0]), rho, gx.reshape(n, 3), gv.reshape(n, 3), float(gnu[0])
 
 
 def oracle_keys(x, cell, table):

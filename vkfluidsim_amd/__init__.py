@@ -15,7 +15,7 @@ from . import _lib
 from ._lib import (BoundaryObject, DeviceArray, Params, Tuning, VfsError, VFS_FLAG_NO_GRAPH, VFS_FLAG_STORE_ACCEL,
                    VFS_FLAG_TRACK_IDS, VFS_NEIGHBOUR_CAPACITY, VFS_SHAPE_BOX, VFS_SHAPE_BOX_WALLS, VFS_SHAPE_SPHERE, call, fptr, lib, uptr)
 
-__all__ = ["Sim", "Batch", "Params", "Tuning", "BoundaryGrid", "SpatialHash", "VfsError", "make_params", "volume_map",
+__all__ = ["Sim", "Batch", "Adjoint", "Params", "Tuning", "BoundaryGrid", "SpatialHash", "VfsError", "make_params", "volume_map",
            "sort_pairs", "scan_exclusive", "spatial_offsets", "reorder_f32x3", "Scene", "lib",
            "VFS_FLAG_TRACK_IDS", "VFS_FLAG_NO_GRAPH", "VFS_FLAG_STORE_ACCEL", "VFS_NEIGHBOUR_CAPACITY",
            "VFS_SHAPE_BOX", "VFS_SHAPE_SPHERE", "VFS_SHAPE_BOX_WALLS", "device_count"]
@@ -383,6 +383,64 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "handle", None):
             lib.vfs_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Adjoint:
+    """Differentiable substeps of a Sim (vfs_adjoint_*): loss = mean squared
+    difference between the density field sampled on a grid and a target."""
+
+    def __init__(self, sim: "Sim", n_substeps: int, checkpoint_every: int, origin, spacing, dims):
+        self.sim = sim
+        self.n = sim.n
+        g = _lib.DensityGrid()
+        g.origin[:] = [float(v) for v in origin]
+        g.spacing[:] = [float(v) for v in spacing]
+        g.dims[:] = [int(v) for v in dims]
+        self.grid = g
+        self.G = int(np.prod([int(v) for v in dims]))
+        h = C.c_void_p()
+        call("vfs_adjoint_create", sim.handle, int(n_substeps), int(checkpoint_every), C.byref(g), C.byref(h))
+        self.handle = h.value
+        self._x0 = DeviceArray(12 * self.n)
+        self._v0 = DeviceArray(12 * self.n)
+        self._target = DeviceArray(4 * self.G)
+        self._density = DeviceArray(4 * self.G)
+        self._gx = DeviceArray(12 * self.n)
+        self._gv = DeviceArray(12 * self.n)
+
+    def forward(self, x0: np.ndarray, v0: np.ndarray, target: np.ndarray | None = None):
+        """(loss, sampled density field) for the initial state (x0, v0)."""
+        for dev, a in ((self._x0, x0), (self._v0, v0)):
+            a = np.ascontiguousarray(a, np.float32).reshape(-1)
+            call("vfs_memcpy_h2d", dev.ptr, a.ctypes.data, a.nbytes)
+        tptr = None
+        if target is not None:
+            t = np.ascontiguousarray(target, np.float32).reshape(-1)
+            call("vfs_memcpy_h2d", self._target.ptr, t.ctypes.data, t.nbytes)
+            tptr = self._target.ptr
+        loss = C.c_double(0)
+        call("vfs_adjoint_forward", self.handle, self._x0.ptr, self._v0.ptr, tptr, self._density.ptr,
+             C.byref(loss))
+        return loss.value, self._density.to_host(np.float32, self.G)
+
+    def backward(self):
+        """(d loss / d x0, d loss / d v0, d loss / d viscosity)."""
+        gnu = C.c_double(0)
+        call("vfs_adjoint_backward", self.handle, self._gx.ptr, self._gv.ptr, C.byref(gnu))
+        gx = self._gx.to_host(np.float32, 3 * self.n).reshape(self.n, 3)
+        gv = self._gv.to_host(np.float32, 3 * self.n).reshape(self.n, 3)
+        return gx, gv, gnu.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.vfs_adjoint_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

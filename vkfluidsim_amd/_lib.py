@@ -77,6 +77,11 @@ class Tuning(C.Structure):
         return (self.sort_bits, self.sort_items, self.block_size, self.stage_rows, self.cell_scale)
 
 
+class DensityGrid(C.Structure):
+    """vfs_density_grid: sample points origin + (i, j, k) * spacing."""
+    _fields_ = [("origin", C.c_float * 3), ("spacing", C.c_float * 3), ("dims", C.c_uint32 * 3)]
+
+
 class DeviceBuffers(C.Structure):
     _fields_ = [
         ("positions", C.c_void_p),
@@ -147,6 +152,10 @@ SIGNATURES = {
     "vfs_batch_autotune": (_i32, [_vp, _u32, _i32, C.POINTER(Tuning), C.POINTER(C.c_float)]),
     "vfs_batch_profile_stages": (_i32, [_vp, _u32, _fp, _up, C.POINTER(C.c_char_p), _u32, _up]),
     "vfs_batch_neighbour_stats": (_i32, [_vp, C.POINTER(C.c_double), _up, _up]),
+    "vfs_adjoint_create": (_i32, [_vp, _u32, _u32, _vp, C.POINTER(_vp)]),
+    "vfs_adjoint_destroy": (_i32, [_vp]),
+    "vfs_adjoint_forward": (_i32, [_vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double)]),
+    "vfs_adjoint_backward": (_i32, [_vp, _vp, _vp, C.POINTER(C.c_double)]),
     "vfs_spatial_hash_create": (_i32, [_u32, C.c_float, C.POINTER(_vp)]),
     "vfs_spatial_hash_set_cell_size": (_i32, [_vp, C.c_float]),
     "vfs_spatial_hash_run": (_i32, [_vp, _vp, _u32, _vp]),
