@@ -15,8 +15,8 @@ namespace vfs {
 int launch_adjoint_substep(const uint32_t* perm, const float4* xs, const float4* vs, const float* rho,
                            const float4* bnd, const uint32_t* nbr, const uint32_t* cnt, const float4* gx_in,
                            const float4* gv_in, float4* abar, float4* gx, float4* gv, float* rbar, float4* gpsi,
-                           uint32_t* dup, float4* gx_out, float4* gv_out, double* nu_bar, uint32_t n,
-                           const SimConsts& c, const DevObject* objs, hipStream_t s);
+                           uint32_t* dup, float4* gx_out, float4* gv_out, double* nu_part, double* nu_bar,
+                           uint32_t n, const SimConsts& c, const DevObject* objs, hipStream_t s);
 int launch_grid_density(const float* origin, const float* spacing, const uint32_t* dims, const float4* xs,
                         const uint32_t* cell_start, const SimConsts& c, double* out, hipStream_t s);
 int launch_grid_loss_grad(const float* origin, const float* spacing, const uint32_t* dims, const float4* x,
@@ -48,7 +48,7 @@ struct vfs_adjoint {
     float4 *abar = nullptr, *gx = nullptr, *gv = nullptr, *gpsi = nullptr;
     float* rbar = nullptr;
     uint32_t *dup = nullptr, *maxc = nullptr;
-    double* nu_bar = nullptr;
+    double *nu_bar = nullptr, *nu_part = nullptr;
     double* rho_grid = nullptr;  // sampled density, summed in double
     float* target = nullptr;
     bool forward_done = false;
@@ -59,7 +59,7 @@ namespace {
 void free_adjoint(vfs_adjoint* a) {
     if (!a) return;
     std::vector<void*> ptrs = {a->gx_a, a->gv_a, a->gx_b, a->gv_b, a->abar, a->gx,       a->gv,
-                               a->gpsi, a->rbar, a->dup,  a->maxc, a->nu_bar, a->rho_grid, a->target};
+                               a->gpsi, a->rbar, a->dup,  a->maxc, a->nu_bar, a->nu_part, a->rho_grid, a->target};
     for (auto* p : a->ck_x) ptrs.push_back(p);
     for (auto* p : a->ck_v) ptrs.push_back(p);
     for (auto& s : a->tape) {
@@ -157,6 +157,7 @@ int vfs_adjoint_create(vfs_sim* sim, uint32_t n_substeps, uint32_t checkpoint_ev
     r |= dev_alloc(&a->dup, n);
     r |= dev_alloc(&a->maxc, 1);
     r |= dev_alloc(&a->nu_bar, 1);
+    r |= dev_alloc(&a->nu_part, ((size_t)n + 255) / 256);
     r |= dev_alloc(&a->rho_grid, G);
     r |= dev_alloc(&a->target, G);
     if (r) {
@@ -260,7 +261,7 @@ int vfs_adjoint_backward(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev,
         for (int k = (int)steps - 1; k >= 0; k--) {
             const auto& t = a->tape[k];
             VFS_TRY(launch_adjoint_substep(t.perm, t.xs, t.vs, t.rho, t.bnd, t.nbr, t.cnt, gx_cur, gv_cur, a->abar,
-                                           a->gx, a->gv, a->rbar, a->gpsi, a->dup, gx_nxt, gv_nxt, a->nu_bar, n, c,
+                                           a->gx, a->gv, a->rbar, a->gpsi, a->dup, gx_nxt, gv_nxt, a->nu_part, a->nu_bar, n, c,
                                            sim->objs_dev, s));
             std::swap(gx_cur, gx_nxt);
             std::swap(gv_cur, gv_nxt);

@@ -233,7 +233,7 @@ struct AdjBuffers {
     uint32_t* dup;        // 1: the particle's stencil keys repeat
     float4* gx_out;       // gx, gv in the order of the substep's input arrays
     float4* gv_out;
-    double* nu_bar;
+    double* nu_part;      // per-block viscosity adjoint of this substep
     uint32_t n;
 };
 
@@ -372,8 +372,23 @@ __global__ __launch_bounds__(256) void k_adj_forces(TapeView t, AdjBuffers a, Si
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int w = 0; w < 256 / 64; w++) s += s_nu[w];
-        if (s != 0.0) atomicAdd(a.nu_bar, s);
+        a.nu_part[blockIdx.x] = s;
     }
+}
+
+// nu_bar += sum of the block partials, in a fixed order (run-to-run identical).
+__global__ __launch_bounds__(256) void k_add_partials(const double* __restrict__ part, uint32_t n,
+                                                      double* __restrict__ total) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256u) acc += part[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t o = 128; o >= 1; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total += s[0];
 }
 
 // Density and boundary volume backwards, then the adjoints of the substep's
@@ -545,16 +560,18 @@ GridDesc to_desc(const float* origin, const float* spacing, const uint32_t* dims
 int launch_adjoint_substep(const uint32_t* perm, const float4* xs, const float4* vs, const float* rho,
                            const float4* bnd, const uint32_t* nbr, const uint32_t* cnt, const float4* gx_in,
                            const float4* gv_in, float4* abar, float4* gx, float4* gv, float* rbar, float4* gpsi,
-                           uint32_t* dup, float4* gx_out, float4* gv_out, double* nu_bar, uint32_t n,
-                           const SimConsts& c, const DevObject* objs, hipStream_t s) {
+                           uint32_t* dup, float4* gx_out, float4* gv_out, double* nu_part, double* nu_bar,
+                           uint32_t n, const SimConsts& c, const DevObject* objs, hipStream_t s) {
     if (n == 0) return 0;
     const TapeView t{perm, xs, vs, rho, bnd, nbr, cnt};
-    const AdjBuffers a{gx_in, gv_in, abar, gx, gv, rbar, gpsi, dup, gx_out, gv_out, nu_bar, n};
+    const AdjBuffers a{gx_in, gv_in, abar, gx, gv, rbar, gpsi, dup, gx_out, gv_out, nu_part, n};
     const uint32_t nb = blocks_for(n, 256);
     hipLaunchKernelGGL(k_adj_integrate, dim3(nb), dim3(256), 0, s, t, a, c);
     VFS_TRY(check_launch("adj_integrate"));
     hipLaunchKernelGGL(k_adj_forces, dim3(nb), dim3(256), 0, s, t, a, c);
     VFS_TRY(check_launch("adj_forces"));
+    hipLaunchKernelGGL(k_add_partials, dim3(1), dim3(256), 0, s, nu_part, nb, nu_bar);
+    VFS_TRY(check_launch("add_partials"));
     hipLaunchKernelGGL(k_adj_density, dim3(nb), dim3(256), 0, s, t, a, c, objs);
     return check_launch("adj_density");
 }
