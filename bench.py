@@ -300,12 +300,13 @@ def bench_adjoint(steps: int, every: int, repeats: int = 3) -> dict:
 
 
 def summarize_stats(db: str) -> str:
-    """Text table of a rocprofv3 --kernel-trace --stats database (top_kernels)."""
+    """Text table of a rocprofv3 --kernel-trace --stats database (top_kernels,
+    whose durations read in microseconds; printed in milliseconds)."""
     import sqlite3
 
     rows = sqlite3.connect(db).execute(
         "select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
-    lines = [f"{'kernel':34s} {'calls':>6s} {'total_us':>11s} {'avg_us':>9s} {'pct':>6s}"]
+    lines = [f"{'kernel':34s} {'calls':>6s} {'total_ms':>11s} {'avg_ms':>9s} {'pct':>6s}"]
     for name, calls, total, avg, pct in rows:
         short = name.replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0].replace("vfs::", "")
         lines.append(f"{short:34s} {calls:6d} {total / 1e3:11.1f} {avg / 1e3:9.3f} {pct:6.2f}")
