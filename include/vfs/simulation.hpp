@@ -142,7 +142,7 @@ public:
     enum class ShapeKind { Box = VFS_SHAPE_BOX, Sphere = VFS_SHAPE_SPHERE, BoxWalls = VFS_SHAPE_BOX_WALLS };
 
     struct ObjectDef {
-        std::string path;           // OBJ resource (reference); not supported here
+        std::string path;           // OBJ resource (the last shape of the file)
         bool analytic{false};
         ShapeKind kind{ShapeKind::Box};
         std::array<double, 6> shape{};  // centre + half extents (or radius)

@@ -311,6 +311,36 @@ int vfs_volume_map_generate(int shape_kind, const double *shape_params, const ui
                             float *sdf_out, float *vm_out, float *box_size, float *box_pos,
                             int threads);
 
+/* ---- triangle meshes as boundary objects -------------------------------
+ * A mesh holds positions and triangles (host), their pseudonormals
+ * (MeshPseudonormals::Build, src/util/mesh_pseudonormals.cpp:11-103) and a
+ * BVH; signed distances (SignedDistanceToMesh, src/util/geometry.cpp:239-286:
+ * closest point in double, sign from the pseudonormal of the closest feature)
+ * are evaluated on the GPU.  The device copy is made on first GPU use. */
+typedef struct vfs_mesh vfs_mesh;
+/* LoadObjMesh (src/util/mesh_loader.cpp:20-113) as generic_scene.cpp:93-94
+ * uses it: the last shape of the file, polygons triangulated. */
+int vfs_mesh_load_obj(const char *path, vfs_mesh **out);
+/* A mesh from positions (3 floats each) and triangles (3 indices each). */
+int vfs_mesh_create(const float *vertices, uint32_t n_vertices, const uint32_t *triangles,
+                    uint32_t n_triangles, vfs_mesh **out);
+/* Counts and the box of the triangles (the BVH root box of mesh_sdf.cpp:33). */
+int vfs_mesh_info(const vfs_mesh *mesh, uint32_t *n_vertices, uint32_t *n_triangles,
+                  float *aabb_min, float *aabb_max);
+/* Copies positions (3 * n_vertices floats) and triangles (3 * n_triangles). */
+int vfs_mesh_get(const vfs_mesh *mesh, float *vertices, uint32_t *triangles);
+/* SignedDistanceToMesh for n host points (3 floats each) into out (n doubles). */
+int vfs_mesh_signed_distance(vfs_mesh *mesh, const float *points, uint32_t n, double *out);
+/* MeshSDF::Build (src/util/mesh_sdf.cpp:28-45) + GenerateVolumeMap
+ * (src/util/volume_map.cpp:9-54): SDF and volume-map grids
+ * (res[0]*res[1]*res[2] floats each, host) over the mesh box grown by
+ * tolerance + margin (the scene loader passes 0 and 8h); box_size/box_pos
+ * receive the grid domain.  One grid node per GPU thread, FP64. */
+int vfs_mesh_volume_map_generate(vfs_mesh *mesh, const uint32_t *res, double support_radius,
+                                 double tolerance, double margin, float *sdf_out, float *vm_out,
+                                 float *box_size, float *box_pos);
+int vfs_mesh_destroy(vfs_mesh *mesh);
+
 /* ---- scenes (src/util/scene_loader.cpp:113-140, src/scenes/generic_scene.cpp) */
 typedef struct vfs_scene vfs_scene;
 /* Parses a scene JSON (reference schema + optional extension keys) and builds

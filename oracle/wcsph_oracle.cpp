@@ -25,11 +25,13 @@
 //   * the counting sort is stable (ascending current index inside a key); the
 //     reference scatter order inside a bucket is nondeterministic.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
 #include <type_traits>
 #include <vector>
@@ -988,6 +990,256 @@ struct KernelD {
     }
 };
 
+// Volume-map value of one grid node (volume_map.cpp:23-52) for any signed
+// distance sdf(const float q[3]) -> double; loops in the reference's order.
+template <class SDF>
+double volume_map_node(const SDF& sdf, const float* x, float support_radius, const std::vector<double>& gx,
+                       const std::vector<double>& gw, const KernelD& kernel) {
+    const double dist = sdf(x);
+    if (dist > 2.0 * support_radius) return 0.0;
+    const double c0 = 0.5 * (double)(support_radius - (-support_radius));
+    const double c1 = 0.5 * (double)(support_radius + (-support_radius));
+    const float sr2 = support_radius * support_radius;
+    double integral = 0.0;
+    for (int i = 0; i < 16; i++) {
+        const double wi = gw[i];
+        for (int j = 0; j < 16; j++) {
+            const double wij = wi * gw[j];
+            for (int k = 0; k < 16; k++) {
+                const double wijk = wij * gw[k];
+                const float xi[3] = {(float)(c0 * gx[i] + c1), (float)(c0 * gx[j] + c1), (float)(c0 * gx[k] + c1)};
+                double val;
+                if ((xi[0] * xi[0] + xi[1] * xi[1]) + xi[2] * xi[2] > sr2) {
+                    val = 0.0;
+                } else {
+                    const float q[3] = {x[0] + xi[0], x[1] + xi[1], x[2] + xi[2]};
+                    const double di = sdf(q);
+                    if (di <= 0.0)
+                        val = 1.0;
+                    else if (di < support_radius)
+                        val = kernel.W(di) / kernel.w_zero;
+                    else
+                        val = 0.0;
+                }
+                integral += wijk * val;
+            }
+        }
+    }
+    integral *= c0 * c0 * c0;
+    return 0.8 * integral;
+}
+
+// ------------------------------------------------------------------ meshes
+// Signed distance to a triangle mesh (geometry.cpp:37-286,
+// mesh_pseudonormals.cpp:11-103), by exhaustive search over the triangles in
+// index order (the reference's BVH only prunes this search; ties keep the
+// first triangle found).
+
+enum class Feature { V0, V1, V2, E01, E12, E02, F };
+
+struct PointTri {
+    double d2;
+    double point[3];
+    Feature feature;
+};
+
+// DistancePointToTriangle (geometry.cpp:37-217): minimise
+// |v0 + s e0 + t e1 - p|^2 over the triangle by Eberly's region classification.
+PointTri point_triangle(const double p[3], const double v0[3], const double v1[3], const double v2[3]) {
+    double diff[3], e0[3], e1[3];
+    for (int a = 0; a < 3; a++) {
+        diff[a] = v0[a] - p[a];
+        e0[a] = v1[a] - v0[a];
+        e1[a] = v2[a] - v0[a];
+    }
+    auto dot = [](const double* u, const double* w) { return (u[0] * w[0] + u[1] * w[1]) + u[2] * w[2]; };
+    const double a00 = dot(e0, e0), a01 = dot(e0, e1), a11 = dot(e1, e1);
+    const double b0 = dot(diff, e0), b1 = dot(diff, e1), c = dot(diff, diff);
+    const double det = std::fabs(a00 * a11 - a01 * a01);
+    double s = a01 * b1 - a11 * b0;
+    double t = a01 * b0 - a00 * b1;
+    double d2 = -1.0;
+    Feature f = Feature::F;
+    const double d2_v1 = a00 + 2.0 * b0 + c, d2_v2 = a11 + 2.0 * b1 + c;
+    auto interior = [&]() { return s * (a00 * s + a01 * t + 2.0 * b0) + t * (a01 * s + a11 * t + 2.0 * b1) + c; };
+    if (s + t <= det) {
+        if (s < 0.0 && t < 0.0 && b0 < 0.0) {  // region 4, toward v1
+            t = 0.0;
+            if (-b0 >= a00) { f = Feature::V1; s = 1.0; d2 = d2_v1; }
+            else { f = Feature::E01; s = -b0 / a00; d2 = b0 * s + c; }
+        } else if (s < 0.0) {  // regions 4 (rest) and 3: edge s = 0
+            s = 0.0;
+            if (b1 >= 0.0) { f = Feature::V0; t = 0.0; d2 = c; }
+            else if (-b1 >= a11) { f = Feature::V2; t = 1.0; d2 = d2_v2; }
+            else { f = Feature::E02; t = -b1 / a11; d2 = b1 * t + c; }
+        } else if (t < 0.0) {  // region 5: edge t = 0
+            t = 0.0;
+            if (b0 >= 0.0) { f = Feature::V0; s = 0.0; d2 = c; }
+            else if (-b0 >= a00) { f = Feature::V1; s = 1.0; d2 = d2_v1; }
+            else { f = Feature::E01; s = -b0 / a00; d2 = b0 * s + c; }
+        } else {  // region 0: interior
+            const double inv_det = 1.0 / det;
+            s *= inv_det;
+            t *= inv_det;
+            f = Feature::F;
+            d2 = interior();
+        }
+    } else if (s < 0.0) {  // region 2
+        const double tmp0 = a01 + b0, tmp1 = a11 + b1;
+        if (tmp1 > tmp0) {
+            const double numer = tmp1 - tmp0, denom = a00 - 2.0 * a01 + a11;
+            if (numer >= denom) { f = Feature::V1; s = 1.0; t = 0.0; d2 = d2_v1; }
+            else { f = Feature::E12; s = numer / denom; t = 1.0 - s; d2 = interior(); }
+        } else {
+            s = 0.0;
+            if (tmp1 <= 0.0) { f = Feature::V2; t = 1.0; d2 = d2_v2; }
+            else if (b1 >= 0.0) { f = Feature::V0; t = 0.0; d2 = c; }
+            else { f = Feature::E02; t = -b1 / a11; d2 = b1 * t + c; }
+        }
+    } else if (t < 0.0) {  // region 6
+        const double tmp0 = a01 + b1, tmp1 = a00 + b0;
+        if (tmp1 > tmp0) {
+            const double numer = tmp1 - tmp0, denom = a00 - 2.0 * a01 + a11;
+            if (numer >= denom) { f = Feature::V2; t = 1.0; s = 0.0; d2 = d2_v2; }
+            else { f = Feature::E12; t = numer / denom; s = 1.0 - t; d2 = interior(); }
+        } else {
+            t = 0.0;
+            if (tmp1 <= 0.0) { f = Feature::V1; s = 1.0; d2 = d2_v1; }
+            else if (b0 >= 0.0) { f = Feature::V0; s = 0.0; d2 = c; }
+            else { f = Feature::E01; s = -b0 / a00; d2 = b0 * s + c; }
+        }
+    } else {  // region 1
+        const double numer = a11 + b1 - a01 - b0;
+        if (numer <= 0.0) { f = Feature::V2; s = 0.0; t = 1.0; d2 = d2_v2; }
+        else {
+            const double denom = a00 - 2.0 * a01 + a11;
+            if (numer >= denom) { f = Feature::V1; s = 1.0; t = 0.0; d2 = d2_v1; }
+            else { f = Feature::E12; s = numer / denom; t = 1.0 - s; d2 = interior(); }
+        }
+    }
+    if (d2 < 0.0) d2 = 0.0;
+    PointTri r;
+    r.d2 = d2;
+    r.feature = f;
+    for (int a = 0; a < 3; a++) r.point[a] = v0[a] + s * e0[a] + t * e1[a];
+    return r;
+}
+
+struct MeshO {
+    std::vector<float> v;       // 3 per vertex
+    std::vector<uint32_t> t;    // 3 per triangle
+    std::vector<float> face;    // 3 per triangle
+    std::vector<float> edge;    // 9 per triangle: edges (0,1), (1,2), (0,2)
+    std::vector<float> vert;    // 3 per vertex
+    float lo[3], hi[3];
+
+    // float vector helpers with glm's evaluation order
+    static void fsub(const float* a, const float* b, float* o) {
+        for (int k = 0; k < 3; k++) o[k] = a[k] - b[k];
+    }
+    static float fdot(const float* a, const float* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+    static void fnormalize(const float* a, float* o) {
+        const float inv = 1.0f / std::sqrt(fdot(a, a));
+        for (int k = 0; k < 3; k++) o[k] = a[k] * inv;
+    }
+
+    MeshO(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt)
+        : v(verts, verts + 3 * (size_t)nv), t(tris, tris + 3 * (size_t)nt) {
+        face.assign(3 * (size_t)nt, 0.0f);
+        edge.assign(9 * (size_t)nt, 0.0f);
+        vert.assign(3 * (size_t)nv, 0.0f);
+        std::map<uint64_t, std::array<float, 3>> sums;  // edge key -> normal sum (stable references)
+        auto slot = [&](uint32_t a, uint32_t b) -> std::array<float, 3>& {
+            const uint64_t key = ((uint64_t)std::min(a, b) << 32) | std::max(a, b);
+            return sums.emplace(key, std::array<float, 3>{0.0f, 0.0f, 0.0f}).first->second;
+        };
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::numeric_limits<float>::max();
+            hi[k] = std::numeric_limits<float>::lowest();
+        }
+        for (uint32_t i = 0; i < nt; i++) {
+            const uint32_t ia = t[3 * i], ib = t[3 * i + 1], ic = t[3 * i + 2];
+            const float *a = &v[3 * (size_t)ia], *b = &v[3 * (size_t)ib], *c = &v[3 * (size_t)ic];
+            for (const float* p : {a, b, c})
+                for (int k = 0; k < 3; k++) {
+                    lo[k] = std::min(lo[k], p[k]);
+                    hi[k] = std::max(hi[k], p[k]);
+                }
+            float ab[3], ac[3], ba[3], bc[3], cb[3], ca[3], n[3];
+            fsub(b, a, ab);
+            fsub(c, a, ac);
+            fsub(a, b, ba);
+            fsub(c, b, bc);
+            fsub(b, c, cb);
+            fsub(a, c, ca);
+            const float cr[3] = {ab[1] * ac[2] - ac[1] * ab[2], ab[2] * ac[0] - ac[2] * ab[0],
+                                 ab[0] * ac[1] - ac[0] * ab[1]};
+            fnormalize(cr, n);
+            for (int k = 0; k < 3; k++) face[3 * (size_t)i + k] = n[k];
+            float u1[3], u2[3];
+            fnormalize(ab, u1);
+            fnormalize(ac, u2);
+            const float al0 = std::acos(std::fabs(fdot(u1, u2)));
+            fnormalize(ba, u1);
+            fnormalize(bc, u2);
+            const float al1 = std::acos(std::fabs(fdot(u1, u2)));
+            fnormalize(cb, u1);
+            fnormalize(ca, u2);
+            const float al2 = std::acos(std::fabs(fdot(u1, u2)));
+            for (int k = 0; k < 3; k++) {
+                vert[3 * (size_t)ia + k] += al0 * n[k];
+                vert[3 * (size_t)ib + k] += al1 * n[k];
+                vert[3 * (size_t)ic + k] += al2 * n[k];
+            }
+            for (auto* e : {&slot(ia, ib), &slot(ib, ic), &slot(ia, ic)})
+                for (int k = 0; k < 3; k++) (*e)[k] += n[k];
+        }
+        for (uint32_t j = 0; j < nv; j++) {
+            float o[3];
+            fnormalize(&vert[3 * (size_t)j], o);
+            for (int k = 0; k < 3; k++) vert[3 * (size_t)j + k] = o[k];
+        }
+        for (uint32_t i = 0; i < nt; i++) {
+            const uint32_t ia = t[3 * i], ib = t[3 * i + 1], ic = t[3 * i + 2];
+            const std::array<float, 3>* es[3] = {&slot(ia, ib), &slot(ib, ic), &slot(ia, ic)};
+            for (int e = 0; e < 3; e++) fnormalize(es[e]->data(), &edge[9 * (size_t)i + 3 * e]);
+        }
+    }
+
+    double signed_distance(const float* q) const {
+        const double p[3] = {q[0], q[1], q[2]};
+        double best = (double)std::numeric_limits<float>::max();
+        uint32_t best_t = 0;
+        PointTri best_r{};
+        const uint32_t nt = (uint32_t)(t.size() / 3);
+        for (uint32_t i = 0; i < nt; i++) {
+            double tv[3][3];
+            for (int c = 0; c < 3; c++)
+                for (int k = 0; k < 3; k++) tv[c][k] = v[3 * (size_t)t[3 * i + c] + k];
+            const PointTri r = point_triangle(p, tv[0], tv[1], tv[2]);
+            if (r.d2 < best) {
+                best = r.d2;
+                best_t = i;
+                best_r = r;
+            }
+        }
+        const float* n;
+        switch (best_r.feature) {
+            case Feature::V0: n = &vert[3 * (size_t)t[3 * best_t]]; break;
+            case Feature::V1: n = &vert[3 * (size_t)t[3 * best_t + 1]]; break;
+            case Feature::V2: n = &vert[3 * (size_t)t[3 * best_t + 2]]; break;
+            case Feature::E01: n = &edge[9 * (size_t)best_t]; break;
+            case Feature::E12: n = &edge[9 * (size_t)best_t + 3]; break;
+            case Feature::E02: n = &edge[9 * (size_t)best_t + 6]; break;
+            default: n = &face[3 * (size_t)best_t]; break;
+        }
+        const float cp[3] = {(float)best_r.point[0], (float)best_r.point[1], (float)best_r.point[2]};
+        const float u[3] = {q[0] - cp[0], q[1] - cp[1], q[2] - cp[2]};
+        const double sign = fdot(u, n) >= 0.0f ? 1.0 : -1.0;
+        return sign * std::sqrt(best);
+    }
+};
+
 }  // namespace orc
 
 using namespace orc;
@@ -1356,6 +1608,71 @@ int orc_volume_map(int kind, const double* params, const uint32_t* res, double s
         const float xf[3] = {(float)ep[0], (float)ep[1], (float)ep[2]};
         sdf_out[idx] = (float)(sh.sdf(xf) - tolerance);
         vm_out[idx] = (float)vm_func(xf);
+    }
+    return 0;
+}
+
+// Pseudonormals of a mesh (face 3, edges 9 per triangle; vertices 3 per vertex).
+int orc_mesh_pseudonormals(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, float* face,
+                           float* edge, float* vert) {
+    const MeshO m(verts, nv, tris, nt);
+    std::copy(m.face.begin(), m.face.end(), face);
+    std::copy(m.edge.begin(), m.edge.end(), edge);
+    std::copy(m.vert.begin(), m.vert.end(), vert);
+    return 0;
+}
+
+// DistancePointToTriangle for n (point, triangle) pairs: d2, closest point, feature index.
+int orc_point_triangle(const double* p, const double* tri9, uint32_t n, double* d2, double* point, int* feature) {
+    for (uint32_t i = 0; i < n; i++) {
+        const PointTri r = point_triangle(p + 3 * (size_t)i, tri9 + 9 * (size_t)i, tri9 + 9 * (size_t)i + 3,
+                                          tri9 + 9 * (size_t)i + 6);
+        d2[i] = r.d2;
+        for (int a = 0; a < 3; a++) point[3 * (size_t)i + a] = r.point[a];
+        feature[i] = (int)r.feature;
+    }
+    return 0;
+}
+
+int orc_mesh_signed_distance(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, const float* pts,
+                             uint32_t n, double* out, int threads) {
+    const MeshO m(verts, nv, tris, nt);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1)
+    for (int64_t i = 0; i < (int64_t)n; i++) out[i] = m.signed_distance(pts + 3 * (size_t)i);
+    return 0;
+}
+
+// SDF + volume-map grids of a mesh: MeshSDF::Build (mesh_sdf.cpp:28-45) and
+// GenerateVolumeMap (volume_map.cpp:9-54) with the exhaustive signed distance.
+int orc_mesh_volume_map(const float* verts, uint32_t nv, const uint32_t* tris, uint32_t nt, const uint32_t* res,
+                        double support_radius_d, double tolerance, double margin, float* sdf_out, float* vm_out,
+                        float* box_size, float* box_pos, int threads) {
+    const MeshO m(verts, nv, tris, nt);
+    const float support_radius = (float)support_radius_d;
+    const double total_margin = tolerance + margin;
+    float dmin[3];
+    double step[3];
+    for (int a = 0; a < 3; a++) {
+        box_size[a] = (m.hi[a] - m.lo[a]) + 2.0f * (float)total_margin;
+        box_pos[a] = m.lo[a] - (float)total_margin;
+        dmin[a] = box_pos[a];
+        const float dmax = box_pos[a] + box_size[a];
+        step[a] = (double)((dmax - dmin[a]) / (float)(res[a] - 1u));
+    }
+    std::vector<double> gx, gw;
+    gauss_legendre(16, gx, gw);
+    const KernelD kernel((double)support_radius);
+    auto sdf = [&](const float* q) { return m.signed_distance(q); };
+    const int64_t total = (int64_t)res[0] * res[1] * res[2];
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads > 0 ? threads : 1)
+    for (int64_t idx = 0; idx < total; idx++) {
+        const uint32_t i = (uint32_t)(idx % res[0]);
+        const uint32_t j = (uint32_t)((idx / res[0]) % res[1]);
+        const uint32_t k = (uint32_t)(idx / ((int64_t)res[0] * res[1]));
+        const float xf[3] = {(float)(step[0] * i + (double)dmin[0]), (float)(step[1] * j + (double)dmin[1]),
+                             (float)(step[2] * k + (double)dmin[2])};
+        sdf_out[idx] = (float)(m.signed_distance(xf) - tolerance);
+        vm_out[idx] = (float)volume_map_node(sdf, xf, support_radius, gx, gw, kernel);
     }
     return 0;
 }

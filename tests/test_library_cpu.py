@@ -66,7 +66,7 @@ def test_sizes_and_workspaces():
 @pytest.mark.parametrize("name,code,needle", [("malformed.json", -4, "parse error"),
                                               ("missing_key.json", -4, "outerWallStiffness"),
                                               ("does_not_exist.json", -4, "cannot open"),
-                                              ("mesh_objects.json", -1, "OBJ mesh")])
+                                              ("mesh_objects.json", -4, "cannot open OBJ file")])
 def test_scene_loader_errors(name, code, needle):
     out = C.c_void_p()
     rc, msg = _err(V.lib.vfs_scene_load, str(SCENES / name).encode(), b"", C.byref(out))

@@ -90,6 +90,72 @@ def volume_map(kind: int, shape_params, resolution, support_radius: float, *, to
                         tuple(float(v) for v in bp))
 
 
+class Mesh:
+    """A triangle mesh boundary object (vfs_mesh_*): OBJ loading on the host,
+    signed distances and boundary grids on the GPU."""
+
+    def __init__(self, handle: int):
+        self.handle = handle
+
+    @classmethod
+    def from_obj(cls, path: str) -> "Mesh":
+        h = C.c_void_p()
+        call("vfs_mesh_load_obj", str(path).encode(), C.byref(h))
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, vertices, triangles) -> "Mesh":
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1)
+        t = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1)
+        h = C.c_void_p()
+        call("vfs_mesh_create", fptr(v), v.size // 3, uptr(t), t.size // 3, C.byref(h))
+        return cls(h.value)
+
+    def info(self):
+        nv, nt = C.c_uint32(), C.c_uint32()
+        lo, hi = np.empty(3, np.float32), np.empty(3, np.float32)
+        call("vfs_mesh_info", self.handle, C.byref(nv), C.byref(nt), fptr(lo), fptr(hi))
+        return nv.value, nt.value, lo, hi
+
+    def arrays(self):
+        nv, nt, _, _ = self.info()
+        v = np.empty(3 * nv, np.float32)
+        t = np.empty(3 * nt, np.uint32)
+        call("vfs_mesh_get", self.handle, fptr(v), uptr(t))
+        return v.reshape(nv, 3), t.reshape(nt, 3)
+
+    def signed_distance(self, points) -> np.ndarray:
+        p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1)
+        out = np.empty(p.size // 3, np.float64)
+        call("vfs_mesh_signed_distance", self.handle, fptr(p), p.size // 3,
+             out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def volume_map(self, resolution, support_radius: float, *, tolerance: float = 0.0,
+                   margin: float | None = None) -> BoundaryGrid:
+        if margin is None:
+            margin = 8.0 * support_radius
+        res = np.ascontiguousarray(np.asarray(resolution, dtype=np.uint32))
+        cnt = int(np.prod(res.astype(np.int64)))
+        sdf, vm = np.empty(cnt, np.float32), np.empty(cnt, np.float32)
+        bs, bp = np.empty(3, np.float32), np.empty(3, np.float32)
+        call("vfs_mesh_volume_map_generate", self.handle, uptr(res), support_radius, tolerance, margin,
+             fptr(sdf), fptr(vm), fptr(bs), fptr(bp))
+        return BoundaryGrid(sdf, vm, tuple(int(r) for r in res), tuple(float(v) for v in bs),
+                            tuple(float(v) for v in bp))
+
+    def close(self) -> None:
+        if self.handle:
+            call("vfs_mesh_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Sim:
     """WCSPHWithBoundaryModel through the C ABI (one scene, one GPU)."""
 

@@ -169,6 +169,13 @@ SIGNATURES = {
     "vfs_reorder_f32x3": (_i32, [_vp, _vp, _vp, _u32, _vp]),
     "vfs_volume_map_generate": (_i32, [_i32, C.POINTER(C.c_double), _up, C.c_double, C.c_double,
                                        C.c_double, _fp, _fp, _fp, _fp, _i32]),
+    "vfs_mesh_load_obj": (_i32, [C.c_char_p, C.POINTER(_vp)]),
+    "vfs_mesh_create": (_i32, [_fp, _u32, _up, _u32, C.POINTER(_vp)]),
+    "vfs_mesh_info": (_i32, [_vp, _up, _up, _fp, _fp]),
+    "vfs_mesh_get": (_i32, [_vp, _fp, _up]),
+    "vfs_mesh_signed_distance": (_i32, [_vp, _fp, _u32, C.POINTER(C.c_double)]),
+    "vfs_mesh_volume_map_generate": (_i32, [_vp, _up, C.c_double, C.c_double, C.c_double, _fp, _fp, _fp, _fp]),
+    "vfs_mesh_destroy": (_i32, [_vp]),
     "vfs_scene_load": (_i32, [C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
     "vfs_scene_sim": (_i32, [_vp, C.POINTER(_vp)]),
     "vfs_scene_reset": (_i32, [_vp]),

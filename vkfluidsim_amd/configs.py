@@ -71,6 +71,84 @@ class Workload:
     params_kw: dict = field(default_factory=dict)
 
 
+def icosphere(subdivisions: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)):
+    """Icosphere mesh: the icosahedron with every triangle split into four
+    `subdivisions` times, midpoints pushed to the sphere (level 4: 2,562
+    vertices, 5,120 triangles).  Triangles are oriented outward.  Returns
+    (float32 vertices (V, 3), uint32 triangles (F, 3))."""
+    g = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, g, 0), (1, g, 0), (-1, -g, 0), (1, -g, 0), (0, -1, g), (0, 1, g), (0, -1, -g), (0, 1, -g),
+         (g, 0, -1), (g, 0, 1), (-g, 0, -1), (-g, 0, 1)]
+    verts = [np.asarray(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                p = verts[a] + verts[b]
+                verts.append(p / np.linalg.norm(p))
+                mid[key] = len(verts) - 1
+            return mid[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.asarray(verts) * radius + np.asarray(center, np.float64)
+    F = np.asarray(faces, np.int64)
+    # outward orientation
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    flip = np.einsum("ij,ij->i", n, V[F].mean(axis=1) - np.asarray(center)) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return V.astype(np.float32), F.astype(np.uint32)
+
+
+def box_mesh(center, half_extents):
+    """Closed box of 12 outward triangles (two per face)."""
+    c, e = np.asarray(center, np.float64), np.asarray(half_extents, np.float64)
+    corners = np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], np.float64)
+    V = c + corners * e
+    quads = [(0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    F = []
+    for a, b, cc, d in quads:
+        F += [(a, b, cc), (a, cc, d)]
+    F = np.asarray(F, np.int64)
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    flip = np.einsum("ij,ij->i", n, V[F].mean(axis=1) - c) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return V.astype(np.float32), F.astype(np.uint32)
+
+
+def torus_mesh(R: float = 1.0, r: float = 0.4, nu: int = 24, nv: int = 12, center=(0.0, 0.0, 0.0)):
+    """Closed torus around the z axis: (nu x nv) quads split into outward
+    triangles (a non-convex test shape)."""
+    u = np.arange(nu) * 2 * np.pi / nu
+    w = np.arange(nv) * 2 * np.pi / nv
+    uu, ww = np.meshgrid(u, w, indexing="ij")
+    pts = np.stack([(R + r * np.cos(ww)) * np.cos(uu), (R + r * np.cos(ww)) * np.sin(uu), r * np.sin(ww)], -1)
+    V = pts.reshape(-1, 3)
+    F = []
+    for i in range(nu):
+        for j in range(nv):
+            a, b = i * nv + j, ((i + 1) % nu) * nv + j
+            c, d = ((i + 1) % nu) * nv + (j + 1) % nv, i * nv + (j + 1) % nv
+            F += [(a, b, c), (a, c, d)]
+    F = np.asarray(F, np.int64)
+    cen = V[F].mean(axis=1)
+    ring = cen.copy()
+    ring[:, 2] = 0
+    ring[:, :2] *= (R / np.linalg.norm(ring[:, :2], axis=1))[:, None]
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    flip = np.einsum("ij,ij->i", n, cen - ring) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return (V + np.asarray(center, np.float64)).astype(np.float32), F.astype(np.uint32)
+
+
 def dam_break(side=(32, 32, 32), spacing: float = 0.02, jitter: float = 0.001, seed: int = SEED_BASE,
               steps: int = 100, map_res: int = 64, name: str = "wcsph_dam_break") -> Workload:
     """Plain WCSPH dam break: lattice block with spacing 0.02 in a 1.6 x 0.8 x 0.8

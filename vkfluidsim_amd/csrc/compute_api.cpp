@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "mesh_internal.h"
 #include "sort_internal.h"
 
 using namespace vfs;
@@ -28,6 +29,10 @@ void destroy_hash(vfs_spatial_hash* h) {
         if (p) (void)hipFree(p);
     delete h;
 }
+
+}  // namespace
+
+namespace vfs {
 
 // Newton iteration on the Legendre polynomial P_n (roots) with the standard
 // weight formula w = 2 / ((1 - x^2) P_n'(x)^2).
@@ -53,7 +58,7 @@ void gauss_legendre(int n, double* x, double* w) {
     }
 }
 
-}  // namespace
+}  // namespace vfs
 
 extern "C" {
 
@@ -187,16 +192,9 @@ int vfs_volume_map_generate(int shape_kind, const double* shape_params, const ui
         lo[a] = (float)(shape_params[a] - e);
         hi[a] = (float)(shape_params[a] + e);
     }
-    const double total_margin = tolerance + margin;
     float dmin[3];
     double step[3];
-    for (int a = 0; a < 3; a++) {
-        box_size[a] = (hi[a] - lo[a]) + 2.0f * (float)total_margin;
-        box_pos[a] = lo[a] - (float)total_margin;
-        dmin[a] = box_pos[a];
-        const float dmax = box_pos[a] + box_size[a];
-        step[a] = (double)((dmax - dmin[a]) / (float)(res[a] - 1u));
-    }
+    grid_domain(lo, hi, tolerance, margin, res, box_size, box_pos, dmin, step);
     double gx[16], gw[16];
     gauss_legendre(16, gx, gw);
     float *d_sdf = nullptr, *d_vm = nullptr;

@@ -359,13 +359,11 @@ GenericScene::GenericScene(const SPHModel::Parameters& b, const WCSPHWithBoundar
     : base(b), model(m), global(g), blocks(std::move(bl)), object_defs(std::move(od)), cell_size(cs), flags(fl) {}
 
 // GenericScene::Init (generic_scene.cpp:21-41) with AddBoundaryObject (:88-124)
-// and CreateBoundaryObjectBuffer (:125-145) for analytic shapes.
+// and CreateBoundaryObjectBuffer (:125-145): OBJ meshes (the file's last
+// shape, generic_scene.cpp:93-94) or analytic shapes.
 void GenericScene::Init() {
     std::vector<WCSPHWithBoundaryModel::BoundaryObject> objs;
     for (const auto& d : object_defs) {
-        if (!d.analytic)
-            throw Error(VFS_ERR_INVALID, "boundary object '" + d.path +
-                                             "': OBJ mesh SDFs are not built; use the analyticShape key");
         WCSPHWithBoundaryModel::BoundaryObject o;
         o.resolution = d.resolution;
         const size_t cnt = (size_t)d.resolution[0] * d.resolution[1] * d.resolution[2];
@@ -373,9 +371,20 @@ void GenericScene::Init() {
         o.volume_map.resize(cnt);
         float bsize[3], bpos[3];
         // tolerance 0 and margin 8h (generic_scene.cpp:99)
-        Check(vfs_volume_map_generate((int)d.kind, d.shape.data(), d.resolution.data(), global.smooth_radius, 0.0,
-                                      8.0 * global.smooth_radius, o.sdf.data(), o.volume_map.data(), bsize, bpos, 0),
-              "vfs_volume_map_generate");
+        if (d.analytic) {
+            Check(vfs_volume_map_generate((int)d.kind, d.shape.data(), d.resolution.data(), global.smooth_radius,
+                                          0.0, 8.0 * global.smooth_radius, o.sdf.data(), o.volume_map.data(), bsize,
+                                          bpos, 0),
+                  "vfs_volume_map_generate");
+        } else {
+            vfs_mesh* mesh = nullptr;
+            Check(vfs_mesh_load_obj(d.path.c_str(), &mesh), "vfs_mesh_load_obj");
+            const int rc = vfs_mesh_volume_map_generate(mesh, d.resolution.data(), global.smooth_radius, 0.0,
+                                                        8.0 * global.smooth_radius, o.sdf.data(),
+                                                        o.volume_map.data(), bsize, bpos);
+            vfs_mesh_destroy(mesh);
+            Check(rc, "vfs_mesh_volume_map_generate");
+        }
         o.box.size = {bsize[0], bsize[1], bsize[2]};
         o.box.pos = {bpos[0], bpos[1], bpos[2]};
         // object matrix M = T * R * S (transform.cpp:35-46); inverse = S^-1 R^T T^-1

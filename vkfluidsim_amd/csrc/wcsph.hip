@@ -14,6 +14,7 @@
 // bucket in ascending sorted index, so sums are accumulated in the reference
 // order.  When the three x-adjacent cells of a row hash to consecutive keys
 // their buckets are contiguous in sorted order and are walked as one range.
+#include "volume_map.h"
 #include "wcsph_math.h"
 
 namespace vfs {
@@ -425,10 +426,6 @@ struct ShapeArgs {
     int kind;
     double p[6];
 };
-struct GLTable {
-    double x[16];
-    double w[16];
-};
 
 __device__ double shape_sdf(const ShapeArgs& s, float qx, float qy, float qz) {
     const double px = qx, py = qy, pz = qz;
@@ -444,22 +441,8 @@ __device__ double shape_sdf(const ShapeArgs& s, float qx, float qy, float qz) {
     return s.kind == VFS_SHAPE_BOX_WALLS ? -d : d;
 }
 
-// Cubic spline of src/util/kernel.cpp:10-34 (FP64).
-__device__ __forceinline__ double kernel_w_f64(double r, double radius, double factor) {
-    const double q = r / radius;
-    if (q <= 1.0) {
-        if (q <= 0.5) {
-            const double q2 = q * q, q3 = q2 * q;
-            return factor * (6.0 * q3 - 6.0 * q2 + 1.0);
-        }
-        return factor * (2.0 * pow(1.0 - q, 3.0));
-    }
-    return 0.0;
-}
-
 // One grid node per thread: SDF value (mesh_sdf.cpp:28-45) and the volume map
-// integral of volume_map.cpp:9-54 with the 16^3 Gauss-Legendre rule of
-// gaussian_quadrature.cpp:2293-2333.
+// integral of volume_map.cpp:9-54 for an analytic shape.
 __global__ __launch_bounds__(128) void k_volume_map(ShapeArgs sh, uint32_t rx, uint32_t ry, uint32_t rz, float sr,
                                                     double tolerance, float d0, float d1, float d2, double s0,
                                                     double s1, double s2, GLTable gl, float* __restrict__ sdf_out,
@@ -467,53 +450,12 @@ __global__ __launch_bounds__(128) void k_volume_map(ShapeArgs sh, uint32_t rx, u
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t total = (uint64_t)rx * ry * rz;
     if (idx >= total) return;
-    const uint32_t i = (uint32_t)(idx % rx);
-    const uint32_t j = (uint32_t)((idx / rx) % ry);
-    const uint32_t k = (uint32_t)(idx / ((uint64_t)rx * ry));
-    const float x0 = (float)(s0 * (double)i + (double)d0);
-    const float x1 = (float)(s1 * (double)j + (double)d1);
-    const float x2 = (float)(s2 * (double)k + (double)d2);
-    sdf_out[idx] = (float)(shape_sdf(sh, x0, x1, x2) - tolerance);
-
-    const double dist = shape_sdf(sh, x0, x1, x2);
-    if (dist > 2.0 * (double)sr) {
-        vm_out[idx] = 0.0f;
-        return;
-    }
-    const double radius = (double)sr;
-    const double factor = 8.0 / (M_PI * radius * radius * radius);
-    const double w_zero = kernel_w_f64(0.0, radius, factor);
-    const double c0 = 0.5 * (double)(sr - (-sr));
-    const double c1 = 0.5 * (double)(sr + (-sr));
-    const float sr2 = sr * sr;
-    double integral = 0.0;
-    for (int a = 0; a < 16; a++) {
-        const double wi = gl.w[a];
-        const float xa = (float)(c0 * gl.x[a] + c1);
-        for (int bb = 0; bb < 16; bb++) {
-            const double wij = wi * gl.w[bb];
-            const float xb = (float)(c0 * gl.x[bb] + c1);
-            for (int cc = 0; cc < 16; cc++) {
-                const double wijk = wij * gl.w[cc];
-                const float xc = (float)(c0 * gl.x[cc] + c1);
-                double val;
-                if ((xa * xa + xb * xb) + xc * xc > sr2) {
-                    val = 0.0;
-                } else {
-                    const double di = shape_sdf(sh, x0 + xa, x1 + xb, x2 + xc);
-                    if (di <= 0.0)
-                        val = 1.0;
-                    else if (di < radius)
-                        val = kernel_w_f64(di, radius, factor) / w_zero;
-                    else
-                        val = 0.0;
-                }
-                integral += wijk * val;
-            }
-        }
-    }
-    integral *= c0 * c0 * c0;
-    vm_out[idx] = (float)(0.8 * integral);
+    float x[3];
+    grid_node(idx, rx, ry, d0, d1, d2, s0, s1, s2, x);
+    const double dist = shape_sdf(sh, x[0], x[1], x[2]);
+    sdf_out[idx] = (float)(dist - tolerance);
+    auto sdf = [&](float qx, float qy, float qz) { return shape_sdf(sh, qx, qy, qz); };
+    vm_out[idx] = (float)volume_map_integral(sdf, x, dist, sr, gl);
 }
 
 inline uint32_t grid_for(uint64_t n, uint32_t block) { return (uint32_t)((n + block - 1) / block); }
