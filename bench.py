@@ -57,6 +57,15 @@ STAGE_KERNELS = {"density": "k_density", "forces_integrate": "k_forces_integrate
 CALIBRATION_MIB = 256
 
 
+def describe_shape(shape) -> str:
+    kind, sp, res = shape
+    grid = "x".join(str(int(r)) for r in res)
+    if kind == "mesh":
+        return f"triangle mesh, {len(sp[0])} vertices / {len(sp[1])} triangles, {grid} grids"
+    names = {0: "analytic box", 1: "analytic sphere", 2: "analytic box walls"}
+    return f"{names.get(kind, str(kind))}, {grid} grids"
+
+
 def kernel_base(name: str) -> str:
     """'void vfs::(anonymous namespace)::k_density<128, true>(...)' -> 'k_density'."""
     name = name.replace("(anonymous namespace)::", "")
@@ -129,7 +138,7 @@ def traffic_probe(workload: str) -> int:
     src.free()
     dst.free()
     w = CF.WORKLOADS[workload]()
-    grids = [V.volume_map(kind, sp, res, w.h) for kind, sp, res in w.shapes]
+    grids = V.boundary_grids(w.shapes, w.h)
     sim = V.Sim(V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos), grids)
     sim.set_state(w.x, w.v)
     sim.substeps(3)
@@ -170,7 +179,7 @@ def bench_batched(n_scenes: int, steps: int, looped_steps: int) -> dict:
     from vkfluidsim_amd import configs as CF
 
     ws = CF.batched_dam_breaks(n_scenes=n_scenes)
-    grids = [[V.volume_map(kind, sp, res, w.h) for kind, sp, res in w.shapes] for w in ws]
+    grids = [V.boundary_grids(w.shapes, w.h) for w in ws]
     params = [V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos) for w in ws]
     total = sum(w.n for w in ws)
 
@@ -252,7 +261,7 @@ def bench_adjoint(steps: int, every: int, repeats: int = 3) -> dict:
     from vkfluidsim_amd import configs as CF
 
     w = CF.dam_break(side=(64, 32, 32), seed=CF.SEED_BASE + 4, steps=steps)
-    grids = [V.volume_map(kind, sp, res, w.h) for kind, sp, res in w.shapes]
+    grids = V.boundary_grids(w.shapes, w.h)
     p = V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos)
     sim = V.Sim(p, grids)
     dims = (32, 32, 32)
@@ -390,7 +399,9 @@ def main() -> int:
         print("bench.py needs a GPU", file=sys.stderr)
         return 1
     w = CF.WORKLOADS[args.workload]()
-    grids = [V.volume_map(kind, sp, res, w.h) for kind, sp, res in w.shapes]
+    t_grid = time.perf_counter()
+    grids = V.boundary_grids(w.shapes, w.h)
+    grid_ms = 1e3 * (time.perf_counter() - t_grid)
     params = V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos)
     sim = V.Sim(params, grids)
     sim.set_state(w.x, w.v)
@@ -444,7 +455,11 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": w.name, "n_particles": w.n, "smooth_radius": w.h, "dt": w.dt,
-                   "boundary_grids": [list(g.resolution) for g in grids], "substep_graph": True},
+                   "boundary_grids": [list(g.resolution) for g in grids],
+                   "boundary_objects": [describe_shape(sh) for sh in w.shapes], "substep_graph": True},
+        "setup": {"boundary_grids_ms": round(grid_ms, 1),
+                  "what": "SDF + volume map of every boundary object generated on the GPU (FP64, 16^3 "
+                          "Gauss-Legendre per node), host copies included"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic.get(dom) if dom in traffic else None,

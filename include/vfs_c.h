@@ -327,6 +327,12 @@ int vfs_mesh_create(const float *vertices, uint32_t n_vertices, const uint32_t *
 /* Counts and the box of the triangles (the BVH root box of mesh_sdf.cpp:33). */
 int vfs_mesh_info(const vfs_mesh *mesh, uint32_t *n_vertices, uint32_t *n_triangles,
                   float *aabb_min, float *aabb_max);
+/* closed: every edge shared by two triangles in opposite directions; acute:
+ * no triangle angle above 90 degrees.  When both hold, the volume-map
+ * generator skips the searches whose outcome the distance at the grid node
+ * already fixes (the signed distance is then 1-Lipschitz); the grids are the
+ * same either way.  No counterpart in the reference. */
+int vfs_mesh_properties(const vfs_mesh *mesh, int *closed, int *acute);
 /* Copies positions (3 * n_vertices floats) and triangles (3 * n_triangles). */
 int vfs_mesh_get(const vfs_mesh *mesh, float *vertices, uint32_t *triangles);
 /* SignedDistanceToMesh for n host points (3 floats each) into out (n doubles). */

@@ -50,6 +50,9 @@ def test_signed_distance_matches_oracle(name):
 @pytest.mark.parametrize("name,res,h", [("box", (12, 10, 14), 0.05), ("icosphere", (14, 14, 14), 0.06),
                                         ("torus", (16, 16, 8), 0.15)])
 def test_mesh_volume_map_matches_oracle(name, res, h):
+    """Box and icosphere (closed, no obtuse angle) take the generator's
+    search-skipping path, the torus (obtuse triangles) the exhaustive one; the
+    oracle searches every point."""
     v, f = MESHES[name]()
     print(f"case {name}: res {res}, h {h}", flush=True)
     m = V.Mesh.from_arrays(v, f)

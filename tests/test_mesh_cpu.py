@@ -204,3 +204,18 @@ def test_icosphere_generator():
     e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
     assert len({tuple(x) for x in e}) == len(e)
     assert {tuple(x) for x in e} == {tuple(x[::-1]) for x in e}
+
+
+def test_mesh_properties():
+    """closed / acute decide whether the volume-map generator may skip searches."""
+    for gen, want in [(lambda: CF.box_mesh((0, 0, 0), (1, 2, 3)), (True, True)),
+                      (lambda: CF.icosphere(3, 0.5), (True, True)),
+                      (lambda: CF.torus_mesh(), (True, False))]:
+        v, f = gen()
+        assert V.Mesh.from_arrays(v, f).properties() == want
+    v, f = CF.icosphere(1, 1.0)
+    assert V.Mesh.from_arrays(v, f[:-1]).properties()[0] is False  # a hole
+    assert V.Mesh.from_arrays(v, f[:, [0, 2, 1]]).properties()[0] is True  # inward, still consistent
+    g = f.copy()
+    g[0] = g[0, [0, 2, 1]]
+    assert V.Mesh.from_arrays(v, g).properties()[0] is False  # one flipped triangle

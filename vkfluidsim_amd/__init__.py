@@ -90,6 +90,20 @@ def volume_map(kind: int, shape_params, resolution, support_radius: float, *, to
                         tuple(float(v) for v in bp))
 
 
+def boundary_grids(shapes, support_radius: float) -> list:
+    """Boundary grids of a workload's shapes: (kind, params, resolution) for an
+    analytic shape, ("mesh", (vertices, triangles), resolution) for a mesh."""
+    out = []
+    for kind, sp, res in shapes:
+        if kind == "mesh":
+            m = Mesh.from_arrays(*sp)
+            out.append(m.volume_map(res, support_radius))
+            m.close()
+        else:
+            out.append(volume_map(kind, sp, res, support_radius))
+    return out
+
+
 class Mesh:
     """A triangle mesh boundary object (vfs_mesh_*): OBJ loading on the host,
     signed distances and boundary grids on the GPU."""
@@ -116,6 +130,12 @@ class Mesh:
         lo, hi = np.empty(3, np.float32), np.empty(3, np.float32)
         call("vfs_mesh_info", self.handle, C.byref(nv), C.byref(nt), fptr(lo), fptr(hi))
         return nv.value, nt.value, lo, hi
+
+    def properties(self):
+        """(closed, acute): see vfs_mesh_properties."""
+        c, a = C.c_int(), C.c_int()
+        call("vfs_mesh_properties", self.handle, C.byref(c), C.byref(a))
+        return bool(c.value), bool(a.value)
 
     def arrays(self):
         nv, nt, _, _ = self.info()

@@ -173,6 +173,7 @@ SIGNATURES = {
     "vfs_mesh_create": (_i32, [_fp, _u32, _up, _u32, C.POINTER(_vp)]),
     "vfs_mesh_info": (_i32, [_vp, _up, _up, _fp, _fp]),
     "vfs_mesh_get": (_i32, [_vp, _fp, _up]),
+    "vfs_mesh_properties": (_i32, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "vfs_mesh_signed_distance": (_i32, [_vp, _fp, _u32, C.POINTER(C.c_double)]),
     "vfs_mesh_volume_map_generate": (_i32, [_vp, _up, C.c_double, C.c_double, C.c_double, _fp, _fp, _fp, _fp]),
     "vfs_mesh_destroy": (_i32, [_vp]),

@@ -162,11 +162,13 @@ def dam_break(side=(32, 32, 32), spacing: float = 0.02, jitter: float = 0.001, s
 
 
 def obstacle_1m(spacing: float = 0.006, map_res: int = 128, side=(128, 64, 128), steps: int = 500,
-                walls: bool = False) -> Workload:
+                walls: bool = False, mesh: bool = True) -> Workload:
     """1,048,576-particle lattice (128 x 64 x 128) in a 2 x 1 x 1 box with a centred
-    sphere obstacle of radius 0.2 (128^3 SDF/volume map), dt = 1e-4, box walls as
-    the penalty force.  The spacing is 0.006 (h = 0.012) so that the block fits
-    the box next to the sphere.
+    obstacle of radius 0.2 (128^3 SDF/volume map), dt = 1e-4, box walls as the
+    penalty force.  The obstacle is an icosphere mesh of subdivision 4 (2,562
+    vertices, 5,120 triangles), or with mesh=False the analytic sphere.  The
+    spacing is 0.006 (h = 0.012) so that the block fits the box next to the
+    sphere.
 
     The lattice starts at 0.8 rho0, so it falls freely (zero pressure) for about
     320 substeps and then hits the penalty floor; with the model's 1/h-scaled
@@ -177,7 +179,10 @@ def obstacle_1m(spacing: float = 0.006, map_res: int = 128, side=(128, 64, 128),
     h = 2.0 * spacing
     x = lattice(side, spacing, (spacing, spacing, spacing), 0.1 * spacing, SEED_BASE + 1)
     v = np.zeros_like(x)
-    shapes = [(1, (1.0, 0.5, 0.5, 0.2), (map_res,) * 3)]
+    if mesh:
+        shapes = [("mesh", icosphere(4, 0.2, (1.0, 0.5, 0.5)), (map_res,) * 3)]
+    else:
+        shapes = [(1, (1.0, 0.5, 0.5, 0.2), (map_res,) * 3)]
     if walls:
         res = tuple(int(np.ceil((e + 16.0 * h) / (0.5 * h))) + 1 for e in (2.0, 1.0, 1.0))
         shapes.append((2, (1.0, 0.5, 0.5, 1.0, 0.5, 0.5), res))

@@ -163,6 +163,27 @@ void build_normals(vfs_mesh& m) {
         for (uint64_t k : keys) add(en.emplace(k, F3{0.0f, 0.0f, 0.0f}).first->second, n);
     }
     for (auto& v : vn) v = normalize(v);
+    // properties that make the signed distance 1-Lipschitz (see MeshView)
+    m.acute = true;
+    for (size_t t = 0; t < nt && m.acute; t++) {
+        const F3 a = P(m.tris[3 * t]), b = P(m.tris[3 * t + 1]), c = P(m.tris[3 * t + 2]);
+        m.acute = dot(normalize(sub(b, a)), normalize(sub(c, a))) >= 0.0f &&
+                  dot(normalize(sub(a, b)), normalize(sub(c, b))) >= 0.0f &&
+                  dot(normalize(sub(b, c)), normalize(sub(a, c))) >= 0.0f;
+    }
+    std::unordered_map<uint64_t, int> directed;
+    auto dkey = [](uint32_t a, uint32_t b) { return ((uint64_t)a << 32) | b; };
+    for (size_t t = 0; t < nt; t++)
+        for (int e = 0; e < 3; e++) directed[dkey(m.tris[3 * t + e], m.tris[3 * t + (e + 1) % 3])]++;
+    m.closed = true;
+    for (const auto& kv : directed) {
+        const uint32_t a = (uint32_t)(kv.first >> 32), b = (uint32_t)kv.first;
+        const auto it = directed.find(dkey(b, a));
+        if (kv.second != 1 || a == b || it == directed.end() || it->second != 1) {
+            m.closed = false;
+            break;
+        }
+    }
     m.normals.resize(nt);
     auto put = [](float* d, F3 v) {
         d[0] = v.x;
@@ -314,6 +335,7 @@ int upload(vfs_mesh* m, MeshView& view) {
         d2 += e * e;
     }
     view.diag2 = d2;
+    view.lipschitz = (m->closed && m->acute) ? 1u : 0u;
     return 0;
 }
 
@@ -365,6 +387,13 @@ int vfs_mesh_info(const vfs_mesh* mesh, uint32_t* n_vertices, uint32_t* n_triang
         if (aabb_min) aabb_min[a] = mesh->lo[a];
         if (aabb_max) aabb_max[a] = mesh->hi[a];
     }
+    return 0;
+}
+
+int vfs_mesh_properties(const vfs_mesh* mesh, int* closed, int* acute) {
+    VFS_CHECK(mesh, VFS_ERR_INVALID, "mesh is null");
+    if (closed) *closed = mesh->closed ? 1 : 0;
+    if (acute) *acute = mesh->acute ? 1 : 0;
     return 0;
 }
 

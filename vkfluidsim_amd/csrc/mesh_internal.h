@@ -35,12 +35,17 @@ struct TriNormals {
 
 // Device view used by the kernels.  centre/diag2 (root box centre, squared
 // diagonal) scale the pruning slack of the closest-point search.
+// lipschitz: the mesh is closed and consistently oriented and no triangle
+// angle exceeds 90 degrees, so the reference's pseudonormals (weights
+// acos|cos|) are the angle-weighted ones and the signed distance they give is
+// the true one, 1-Lipschitz in the query point.
 struct MeshView {
     const MeshNode* nodes;
     const float4* tri;  // 3 per triangle: v0, v1, v2 (xyz)
     const TriNormals* pn;
     float centre[3];
     double diag2;
+    uint32_t lipschitz;
 };
 
 // Grid domain of a boundary object (mesh_sdf.cpp:33-38): the object's box grown
@@ -77,6 +82,8 @@ struct vfs_mesh {
     std::vector<vfs::TriNormals> normals;  // input order
     uint32_t depth = 0;
     float lo[3]{}, hi[3]{};
+    bool closed = false;  // every edge in two triangles, traversed in opposite directions
+    bool acute = false;   // no triangle angle above 90 degrees
     // device copy
     vfs::MeshNode* d_nodes = nullptr;
     float4* d_tri = nullptr;

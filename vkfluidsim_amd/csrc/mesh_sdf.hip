@@ -176,11 +176,34 @@ struct Closest {
     uint32_t tri;
 };
 
-// Minimum over all triangles of the computed squared distance (ties: first
-// found).  A subtree is skipped only when its box is farther than the best
-// distance plus a slack far above the rounding of the triangle distances, so
-// no triangle whose computed distance beats the best is ever skipped.
-__device__ Closest mesh_closest(const MeshView& m, float qx, float qy, float qz) {
+// Distance from q to triangle k (BVH order); keeps it in `best` if closer.
+__device__ __forceinline__ void try_triangle(const MeshView& m, const double* q, uint32_t k, Closest& best) {
+    const float4 a = m.tri[3 * (size_t)k], b = m.tri[3 * (size_t)k + 1], cc = m.tri[3 * (size_t)k + 2];
+    const double v0[3] = {a.x, a.y, a.z};
+    const double e0[3] = {(double)b.x - v0[0], (double)b.y - v0[1], (double)b.z - v0[2]};
+    const double e1[3] = {(double)cc.x - v0[0], (double)cc.y - v0[1], (double)cc.z - v0[2]};
+    const TriDistance td = tri_distance(q, v0, e0, e1);
+    if (td.d2 < best.d2) {
+        best.d2 = td.d2;
+        best.entity = td.entity;
+        best.tri = k;
+        // closest point v0 + s e0 + t e1, stored as float (ClosestPointQueryResult)
+        best.p[0] = (float)(v0[0] + td.s * e0[0] + td.t * e1[0]);
+        best.p[1] = (float)(v0[1] + td.s * e0[1] + td.t * e1[1]);
+        best.p[2] = (float)(v0[2] + td.s * e0[2] + td.t * e1[2]);
+    }
+}
+
+constexpr uint32_t kNoHint = 0xffffffffu;
+
+// Minimum over all triangles of the computed squared distance.  `hint` (a
+// triangle near the query, e.g. the previous query's closest) is tried first
+// so that the search starts with a tight bound; among exactly equal
+// distances the first found is kept.  A subtree is skipped only when its box
+// is farther than the best distance plus a slack far above the rounding of
+// the triangle distances, so no triangle whose computed distance beats the
+// best is ever skipped.
+__device__ Closest mesh_closest(const MeshView& m, float qx, float qy, float qz, uint32_t hint = kNoHint) {
     const double q[3] = {qx, qy, qz};
     const double cx = q[0] - m.centre[0], cy = q[1] - m.centre[1], cz = q[2] - m.centre[2];
     const double slack = 1e-12 * (m.diag2 + (cx * cx + cy * cy) + cz * cz);
@@ -189,6 +212,7 @@ __device__ Closest mesh_closest(const MeshView& m, float qx, float qy, float qz)
     best.p[0] = best.p[1] = best.p[2] = 0.0f;
     best.entity = kF;
     best.tri = 0;
+    if (hint != kNoHint) try_triangle(m, q, hint, best);
     uint32_t stack[kMeshStack];
     int sp = 0;
     uint32_t node = 0;
@@ -196,22 +220,7 @@ __device__ Closest mesh_closest(const MeshView& m, float qx, float qy, float qz)
         const MeshNode nd = m.nodes[node];
         bool descend = false;
         if (nd.count) {
-            for (uint32_t k = nd.first; k < nd.first + nd.count; k++) {
-                const float4 a = m.tri[3 * (size_t)k], b = m.tri[3 * (size_t)k + 1], cc = m.tri[3 * (size_t)k + 2];
-                const double v0[3] = {a.x, a.y, a.z};
-                const double e0[3] = {(double)b.x - v0[0], (double)b.y - v0[1], (double)b.z - v0[2]};
-                const double e1[3] = {(double)cc.x - v0[0], (double)cc.y - v0[1], (double)cc.z - v0[2]};
-                const TriDistance td = tri_distance(q, v0, e0, e1);
-                if (td.d2 < best.d2) {
-                    best.d2 = td.d2;
-                    best.entity = td.entity;
-                    best.tri = k;
-                    // closest point v0 + s e0 + t e1, stored as float (ClosestPointQueryResult)
-                    best.p[0] = (float)(v0[0] + td.s * e0[0] + td.t * e1[0]);
-                    best.p[1] = (float)(v0[1] + td.s * e0[1] + td.t * e1[1]);
-                    best.p[2] = (float)(v0[2] + td.s * e0[2] + td.t * e1[2]);
-                }
-            }
+            for (uint32_t k = nd.first; k < nd.first + nd.count; k++) try_triangle(m, q, k, best);
         } else {
             const uint32_t a = nd.first, b = nd.first + 1u;
             const double da = box_d2(m.nodes[a], q), db = box_d2(m.nodes[b], q);
@@ -244,9 +253,11 @@ __device__ Closest mesh_closest(const MeshView& m, float qx, float qy, float qz)
 }
 
 // SignedDistanceToMesh (geometry.cpp:239-286): the distance, signed by the
-// pseudonormal of the closest feature against (query - closest point) in float.
-__device__ double mesh_signed_distance(const MeshView& m, float qx, float qy, float qz) {
-    const Closest r = mesh_closest(m, qx, qy, qz);
+// pseudonormal of the closest feature against (query - closest point) in
+// float.  `hint` is updated to the closest triangle.
+__device__ double mesh_signed_distance(const MeshView& m, float qx, float qy, float qz, uint32_t& hint) {
+    const Closest r = mesh_closest(m, qx, qy, qz, hint);
+    hint = r.tri;
     const TriNormals& pn = m.pn[r.tri];
     const float* n;
     switch (r.entity) {
@@ -268,10 +279,17 @@ __global__ __launch_bounds__(256) void k_mesh_signed_distance(MeshView m, const 
                                                               double* __restrict__ out) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
-    out[i] = mesh_signed_distance(m, pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]);
+    uint32_t hint = kNoHint;
+    out[i] = mesh_signed_distance(m, pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2], hint);
 }
 
-// One grid node per thread: the SDF value and the volume-map integral.
+// One grid node per thread: the SDF value and the volume-map integral
+// (volume_map.cpp:23-52, the reference's loop order and expressions).  When
+// the signed distance is 1-Lipschitz (MeshView::lipschitz), a quadrature
+// point x + xi with d(x) + |xi| < -delta is inside (value 1) and one with
+// d(x) - |xi| > radius + delta has value 0, whatever the search would find;
+// delta covers the float rounding of x + xi.  Every other point is searched,
+// starting from the previous point's closest triangle.
 __global__ __launch_bounds__(128) void k_mesh_volume_map(MeshView m, uint32_t rx, uint32_t ry, uint32_t rz, float sr,
                                                          double tolerance, float d0, float d1, float d2, double s0,
                                                          double s1, double s2, GLTable gl,
@@ -281,10 +299,57 @@ __global__ __launch_bounds__(128) void k_mesh_volume_map(MeshView m, uint32_t rx
     if (idx >= total) return;
     float x[3];
     grid_node(idx, rx, ry, d0, d1, d2, s0, s1, s2, x);
-    const double dist = mesh_signed_distance(m, x[0], x[1], x[2]);
+    uint32_t hint = kNoHint;
+    const double dist = mesh_signed_distance(m, x[0], x[1], x[2], hint);
     sdf_out[idx] = (float)(dist - tolerance);
-    auto sdf = [&](float qx, float qy, float qz) { return mesh_signed_distance(m, qx, qy, qz); };
-    vm_out[idx] = (float)volume_map_integral(sdf, x, dist, sr, gl);
+    if (dist > 2.0 * (double)sr) {
+        vm_out[idx] = 0.0f;
+        return;
+    }
+    const double radius = (double)sr;
+    const double factor = 8.0 / (M_PI * radius * radius * radius);
+    const double w_zero = kernel_w_f64(0.0, radius, factor);
+    const double c0 = 0.5 * (double)(sr - (-sr));
+    const double c1 = 0.5 * (double)(sr + (-sr));
+    const float sr2 = sr * sr;
+    const double xmax = fmax(fabs((double)x[0]), fmax(fabs((double)x[1]), fabs((double)x[2])));
+    const double delta = 8.0 * 1.1920928955078125e-07 * (xmax + radius) + 1e-12;
+    const bool lip = m.lipschitz != 0u;
+    double integral = 0.0;
+    for (int a = 0; a < 16; a++) {
+        const double wi = gl.w[a];
+        const float xa = (float)(c0 * gl.x[a] + c1);
+        for (int bb = 0; bb < 16; bb++) {
+            const double wij = wi * gl.w[bb];
+            const float xb = (float)(c0 * gl.x[bb] + c1);
+            for (int cc = 0; cc < 16; cc++) {
+                const double wijk = wij * gl.w[cc];
+                const float xc = (float)(c0 * gl.x[cc] + c1);
+                double val;
+                if ((xa * xa + xb * xb) + xc * xc > sr2) {
+                    val = 0.0;
+                } else {
+                    const double r = sqrt((double)xa * xa + (double)xb * xb + (double)xc * xc);
+                    if (lip && dist + r < -delta) {
+                        val = 1.0;
+                    } else if (lip && dist - r > radius + delta) {
+                        val = 0.0;
+                    } else {
+                        const double di = mesh_signed_distance(m, x[0] + xa, x[1] + xb, x[2] + xc, hint);
+                        if (di <= 0.0)
+                            val = 1.0;
+                        else if (di < radius)
+                            val = kernel_w_f64(di, radius, factor) / w_zero;
+                        else
+                            val = 0.0;
+                    }
+                }
+                integral += wijk * val;
+            }
+        }
+    }
+    integral *= c0 * c0 * c0;
+    vm_out[idx] = (float)(0.8 * integral);
 }
 
 inline uint32_t blocks_of(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
