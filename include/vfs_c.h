@@ -290,6 +290,30 @@ int vfs_spatial_offsets(const uint32_t *sorted_keys_dev, uint32_t *offsets_dev, 
 int vfs_reorder_f32x3(const uint32_t *sorted_indices_dev, const float *src_dev, float *dst_dev,
                       uint32_t n, void *stream);
 
+/* ---- Lague (Clavet double-density) model ------------------------------
+ * LagueModel (src/models/lague_model.h:17-24, lague_model.cpp:87-125): per
+ * substep, external forces with predicted positions x + v / 120, the spatial
+ * hash of the predictions, density and near density (spiky kernels),
+ * pressure, near pressure and Poly6 viscosity, integration and wall
+ * collisions with damping.  The handle is a vfs_sim: vfs_sim_step /
+ * substeps / set_state / get_state / get_hash / set_params / destroy apply
+ * (vfs_params' WCSPH fields are unused).  One reading differs from the
+ * reference: its pressure kernel reads neighbour velocities while other
+ * invocations of the same dispatch update theirs; here every read sees the
+ * velocities from before the dispatch. */
+typedef struct vfs_lague_params {
+    float wall_damping_factor;
+    float pressure_multiplier;
+    float near_pressure_multiplier;
+    float viscosity_strenght;
+} vfs_lague_params;
+/* LagueModel(base, par) + Init; lague == NULL selects the reference's
+ * defaults {0.95, 288, 2.16, 0.03} (lague_model.cpp:26-34). */
+int vfs_lague_create(const vfs_params *params, const vfs_lague_params *lague, vfs_sim **out);
+/* The parameter sliders of LagueModel::DrawDebugUI (lague_model.cpp:131-154). */
+int vfs_lague_set_params(vfs_sim *sim, const vfs_lague_params *lague);
+int vfs_lague_get_params(const vfs_sim *sim, vfs_lague_params *lague);
+
 /* ---- boundary grids (host precompute, src/util/volume_map.cpp:9-54) --- */
 
 /* Analytic shapes whose signed distance the grid generator evaluates. */
@@ -346,6 +370,33 @@ int vfs_mesh_volume_map_generate(vfs_mesh *mesh, const uint32_t *res, double sup
                                  double tolerance, double margin, float *sdf_out, float *vm_out,
                                  float *box_size, float *box_pos);
 int vfs_mesh_destroy(vfs_mesh *mesh);
+
+/* ---- trajectory export ----------------------------------------------------
+ * The "record particle and object positions, and export to a file" item of
+ * the reference's to-do list (README.md:33; no counterpart in its code).
+ * File layout, little endian:
+ *   header, 64 bytes: char magic[8] = "VFSTRAJ1"; uint32 version = 1;
+ *     uint32 n_particles; uint32 fields (VFS_TRAJ_* bits); uint32 n_objects;
+ *     uint64 n_frames (written by vfs_trajectory_close); float box_pos[3];
+ *     float box_size[3]; uint32 reserved[2];
+ *   n_objects x float[16]: each boundary object's world-to-object matrix
+ *     (column-major, as vfs_boundary_object.transform);
+ *   frames: uint64 substep; double time; then, for each field present in
+ *     this order, positions float[3 n], velocities float[3 n], densities
+ *     float[n], particle ids uint32[n] (particles in the simulation's current
+ *     order; the ids map them to their initial index). */
+enum {
+    VFS_TRAJ_POSITIONS = 1u,
+    VFS_TRAJ_VELOCITIES = 2u,
+    VFS_TRAJ_DENSITIES = 4u,
+    VFS_TRAJ_IDS = 8u /* needs VFS_FLAG_TRACK_IDS */
+};
+typedef struct vfs_trajectory vfs_trajectory;
+int vfs_trajectory_open(vfs_sim *sim, const char *path, uint32_t fields, vfs_trajectory **out);
+/* Appends the simulation's current state (after the work queued so far). */
+int vfs_trajectory_record(vfs_trajectory *trajectory, uint64_t substep, double time);
+/* Writes the frame count into the header and closes the file. */
+int vfs_trajectory_close(vfs_trajectory *trajectory);
 
 /* ---- scenes (src/util/scene_loader.cpp:113-140, src/scenes/generic_scene.cpp) */
 typedef struct vfs_scene vfs_scene;

@@ -1029,6 +1029,248 @@ double volume_map_node(const SDF& sdf, const float* x, float support_radius, con
     return 0.8 * integral;
 }
 
+// ------------------------------------------------------------ Lague model
+// LagueModel (src/models/lague_model.cpp:87-125, lague_model.slang), the
+// Clavet double-density model: per substep ExternalForces (predicted
+// positions), the spatial hash of the predicted positions with x, v and the
+// predictions reordered, CalculateDensities, CalculatePressureForces and
+// UpdatePositions.  Canonical readings as for the WCSPH restatement (fused
+// dot products, vector / scalar as a multiplication by the reciprocal,
+// integral pow by binary powering in double), and one more: the reference's
+// pressure kernel reads neighbour velocities that other invocations of the
+// same dispatch are updating (a race); here they are read as they were
+// before the dispatch.
+struct LagueParamsC {  // identical layout to vfs_lague_params
+    float wall_damping_factor;
+    float pressure_multiplier;
+    float near_pressure_multiplier;
+    float viscosity_strenght;
+};
+
+template <class R>
+struct LagueSim {
+    R g[3], h, cell, dt, pf, rho0, pm, npm, visc, damp;
+    R sp2, sp3, dsp2, dsp3, poly6;
+    R bb_lo[3], bb_hi[3];
+    uint32_t n = 0;
+    int threads = 1;
+    std::vector<R> x, v, pred, rho, near_rho, nv;
+    std::vector<uint32_t> pid, keys, sorted_keys, perm, offsets, counts;
+
+    static R ipow(R x, int e) {  // canonical pow with an integral exponent
+        double base = (double)x, result = 1.0;
+        unsigned k = (unsigned)e;
+        while (k) {
+            if (k & 1u) result *= base;
+            base *= base;
+            k >>= 1;
+        }
+        return (R)result;
+    }
+
+    void init(const ParamsC& p, const LagueParamsC& lp) {
+        for (int c = 0; c < 3; c++) g[c] = p.gravity[c];
+        h = p.smooth_radius;
+        cell = p.cell_size > 0.0f ? p.cell_size : p.smooth_radius;
+        dt = (R)(p.time_scale * p.fixed_dt / (float)p.iterations);
+        pf = (R)(1.0f / 120.0f);  // prediction_factor, lague_model.slang:218
+        rho0 = p.target_density;
+        pm = lp.pressure_multiplier;
+        npm = lp.near_pressure_multiplier;
+        visc = lp.viscosity_strenght;
+        damp = lp.wall_damping_factor;
+        // CalcKernelCoefficients (model.cpp:224-233): glm::pi<float>() and std::pow(float, int) in double
+        const float pi_f = 3.14159265358979323846f;
+        const float r = p.smooth_radius;
+        sp3 = 15.0f / (pi_f * (float)std::pow((double)r, 6));
+        sp2 = 15.0f / (2.0f * pi_f * (float)std::pow((double)r, 5));
+        dsp3 = 45.0f / (pi_f * (float)std::pow((double)r, 6));
+        dsp2 = 15.0f / (pi_f * (float)std::pow((double)r, 5));
+        // Poly6 (kernels_3d.slang:16-24): 315 / (64 PI pow(|r|, 9)) with the shader's PI
+        const float pi_shader = 3.1415926f;
+        poly6 = (R)(315.0f / ((64.0f * pi_shader) * (float)ipow((float)std::fabs(r), 9)));
+        for (int c = 0; c < 3; c++) {
+            bb_lo[c] = p.bbox_pos[c];
+            bb_hi[c] = (R)(p.bbox_pos[c] + p.bbox_size[c]);
+        }
+        n = (uint32_t)p.n_particles;
+        x.assign(3 * (size_t)n, R(0));
+        v.assign(3 * (size_t)n, R(0));
+        pred.assign(3 * (size_t)n, R(0));
+        nv.assign(3 * (size_t)n, R(0));
+        rho.assign(n, R(0));
+        near_rho.assign(n, R(0));
+        pid.resize(n);
+        for (uint32_t i = 0; i < n; i++) pid[i] = i;
+        keys.assign(n, 0u);
+        sorted_keys.assign(n, 0u);
+        perm.assign(n, 0u);
+        offsets.assign(n, 0u);
+        counts.assign(n, 0u);
+    }
+
+    // kernels_3d.slang:26-60
+    R spiky3(R d) const { return d < h ? (((h - d) * (h - d)) * (h - d)) * sp3 : R(0); }
+    R spiky2(R d) const { return d < h ? ((h - d) * (h - d)) * sp2 : R(0); }
+    R grad_spiky3(R d) const { return d <= h ? ((-(h - d)) * (h - d)) * dsp3 : R(0); }
+    R grad_spiky2(R d) const { return d <= h ? (-(h - d)) * dsp2 : R(0); }
+    R poly6_w(R d) const {
+        if (d < h) {
+            const R w = h * h - d * d;
+            return ((w * w) * w) * poly6;
+        }
+        return R(0);
+    }
+
+    template <class F>
+    void for_neighbours(const R* pos, F&& f) const {
+        int32_t oc[3];
+        get_cell<R>(pos, cell, oc);
+        for (int o = 0; o < 27; o++) {
+            const uint32_t key =
+                hash_cell(oc[0] + kOffsets3D[o][0], oc[1] + kOffsets3D[o][1], oc[2] + kOffsets3D[o][2]) % n;
+            uint32_t cur = offsets[key];
+            while (cur < n) {
+                const uint32_t j = cur++;
+                if (!f(j, key)) break;
+            }
+        }
+    }
+
+    void substep() {
+        if (n == 0) return;
+        // ExternalForces (slang:211-221)
+        for (uint32_t i = 0; i < n; i++)
+            for (int c = 0; c < 3; c++) {
+                v[3 * i + c] = v[3 * i + c] + g[c] * dt;
+                pred[3 * i + c] = x[3 * i + c] + v[3 * i + c] * pf;
+            }
+        // spatial hash of the predicted positions; reorder of x, v and the predictions
+        for (uint32_t i = 0; i < n; i++) {
+            int32_t c[3];
+            get_cell<R>(&pred[3 * i], cell, c);
+            keys[i] = hash_cell(c[0], c[1], c[2]) % n;
+        }
+        std::fill(counts.begin(), counts.end(), 0u);
+        for (uint32_t i = 0; i < n; i++) counts[keys[i]]++;
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < n; k++) {
+            const uint32_t c = counts[k];
+            counts[k] = run;
+            run += c;
+        }
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t d = counts[keys[i]]++;
+            sorted_keys[d] = keys[i];
+            perm[d] = i;
+        }
+        std::fill(offsets.begin(), offsets.end(), n);
+        for (uint32_t i = 0; i < n; i++)
+            if (i == 0 || sorted_keys[i] != sorted_keys[i - 1]) offsets[sorted_keys[i]] = i;
+        {
+            std::vector<R> tx(x.size()), tv(v.size()), tp(pred.size());
+            std::vector<uint32_t> tid(n);
+            for (uint32_t i = 0; i < n; i++) {
+                const uint32_t s = perm[i];
+                for (int c = 0; c < 3; c++) {
+                    tx[3 * i + c] = x[3 * s + c];
+                    tv[3 * i + c] = v[3 * s + c];
+                    tp[3 * i + c] = pred[3 * s + c];
+                }
+                tid[i] = pid[s];
+            }
+            x.swap(tx);
+            v.swap(tv);
+            pred.swap(tp);
+            pid.swap(tid);
+        }
+        const R h2 = h * h;
+        // CalculateDensities (slang:60-97): self included
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+        for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+            const uint32_t i = (uint32_t)ii;
+            const R* pos = &pred[3 * i];
+            R density = R(0), near_density = R(0);
+            for_neighbours(pos, [&](uint32_t j, uint32_t key) {
+                if (sorted_keys[j] != key) return false;
+                R d[3];
+                for (int c = 0; c < 3; c++) d[c] = pred[3 * j + c] - pos[c];
+                const R r2 = dot3<R>(d, d);
+                if (r2 > h2) return true;
+                const R dst = std::sqrt(r2);
+                density += spiky2(dst);
+                near_density += spiky3(dst);
+                return true;
+            });
+            rho[i] = density;
+            near_rho[i] = near_density;
+        }
+        // CalculatePressureForces (slang:99-160), velocities from before the dispatch
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+        for (int64_t ii = 0; ii < (int64_t)n; ii++) {
+            const uint32_t i = (uint32_t)ii;
+            const R density = rho[i], density_near = near_rho[i];
+            const R pressure = (density - rho0) * pm;
+            const R near_pressure = npm * density_near;
+            const R* vel = &v[3 * i];
+            const R* pos = &pred[3 * i];
+            R pf3[3] = {0, 0, 0}, vf3[3] = {0, 0, 0};
+            for_neighbours(pos, [&](uint32_t j, uint32_t key) {
+                if (j == i) return true;
+                if (sorted_keys[j] != key) return false;
+                R off[3];
+                for (int c = 0; c < 3; c++) off[c] = pred[3 * j + c] - pos[c];
+                const R r2 = dot3<R>(off, off);
+                if (r2 > h2) return true;
+                const R nd = rho[j], nnd = near_rho[j];
+                const R np = (nd - rho0) * pm;
+                const R nnp = npm * nnd;
+                const R shared = (pressure + np) * R(0.5);
+                const R shared_near = (near_pressure + nnp) * R(0.5);
+                const R dst = std::sqrt(r2);
+                R dir[3];
+                if (dst > R(0)) {
+                    const R inv = R(1) / dst;
+                    for (int c = 0; c < 3; c++) dir[c] = off[c] * inv;
+                } else {
+                    dir[0] = R(0);
+                    dir[1] = R(1);
+                    dir[2] = R(0);
+                }
+                const R dd = grad_spiky2(dst), ndd = grad_spiky3(dst);
+                const R inv_nd = R(1) / nd, inv_nnd = R(1) / nnd;
+                for (int c = 0; c < 3; c++) pf3[c] += ((dir[c] * dd) * shared) * inv_nd;
+                for (int c = 0; c < 3; c++) pf3[c] += ((dir[c] * ndd) * shared_near) * inv_nnd;
+                const R w = poly6_w(dst);
+                for (int c = 0; c < 3; c++) vf3[c] += (v[3 * j + c] - vel[c]) * w;
+                return true;
+            });
+            const R inv_density = R(1) / density;
+            for (int c = 0; c < 3; c++) {
+                const R acc = pf3[c] * inv_density + vf3[c] * visc;
+                nv[3 * i + c] = vel[c] + acc * dt;
+            }
+        }
+        v.swap(nv);
+        // UpdatePositions + ResolveCollisions (slang:232-250, 162-200)
+        for (uint32_t i = 0; i < n; i++) {
+            R* pos = &x[3 * i];
+            R* vel = &v[3 * i];
+            for (int c = 0; c < 3; c++) pos[c] = pos[c] + vel[c] * dt;
+            for (int c = 0; c < 3; c++)
+                if (pos[c] < bb_lo[c]) {
+                    pos[c] = bb_lo[c];
+                    vel[c] = (-vel[c]) * damp;
+                }
+            for (int c = 0; c < 3; c++)
+                if (pos[c] > bb_hi[c]) {
+                    pos[c] = bb_hi[c];
+                    vel[c] = (-vel[c]) * damp;
+                }
+        }
+    }
+};
+
 // ------------------------------------------------------------------ meshes
 // Signed distance to a triangle mesh (geometry.cpp:37-286,
 // mesh_pseudonormals.cpp:11-103), by exhaustive search over the triangles in
@@ -1675,6 +1917,75 @@ int orc_mesh_volume_map(const float* verts, uint32_t nv, const uint32_t* tris, u
         vm_out[idx] = (float)volume_map_node(sdf, xf, support_radius, gx, gw, kernel);
     }
     return 0;
+}
+
+// Lague model: handle of an FP32 (fp64 = 0) or FP64 restatement.
+struct OrcLague {
+    int fp64 = 0;
+    LagueSim<float> s32;
+    LagueSim<double> s64;
+};
+
+void* orc_lague_create(const ParamsC* p, const LagueParamsC* lp, int fp64, int threads) {
+    if (!p || !lp || p->n_particles < 0) return nullptr;
+    auto* s = new OrcLague();
+    s->fp64 = fp64;
+    if (fp64) {
+        s->s64.init(*p, *lp);
+        s->s64.threads = threads < 1 ? 1 : threads;
+    } else {
+        s->s32.init(*p, *lp);
+        s->s32.threads = threads < 1 ? 1 : threads;
+    }
+    return s;
+}
+void orc_lague_destroy(void* h) { delete (OrcLague*)h; }
+
+void orc_lague_set_state(void* h, const float* x, const float* v) {
+    auto* s = (OrcLague*)h;
+    auto set = [&](auto& sim) {
+        for (size_t i = 0; i < 3 * (size_t)sim.n; i++) {
+            sim.x[i] = x[i];
+            sim.v[i] = v[i];
+        }
+        for (uint32_t i = 0; i < sim.n; i++) sim.pid[i] = i;
+    };
+    if (s->fp64) set(s->s64); else set(s->s32);
+}
+
+void orc_lague_substeps(void* h, uint32_t k) {
+    auto* s = (OrcLague*)h;
+    for (uint32_t i = 0; i < k; i++) {
+        if (s->fp64) s->s64.substep(); else s->s32.substep();
+    }
+}
+
+// x, v (3 n), densities, near densities (n), ids (n); any may be null
+void orc_lague_get_state(void* h, double* x, double* v, double* rho, double* near_rho, uint32_t* pid) {
+    auto* s = (OrcLague*)h;
+    auto get = [&](auto& sim) {
+        for (size_t i = 0; i < 3 * (size_t)sim.n; i++) {
+            if (x) x[i] = (double)sim.x[i];
+            if (v) v[i] = (double)sim.v[i];
+        }
+        for (uint32_t i = 0; i < sim.n; i++) {
+            if (rho) rho[i] = (double)sim.rho[i];
+            if (near_rho) near_rho[i] = (double)sim.near_rho[i];
+            if (pid) pid[i] = sim.pid[i];
+        }
+    };
+    if (s->fp64) get(s->s64); else get(s->s32);
+}
+
+// Kernel values of the FP32 model (spiky2, spiky3, grad_spiky2, grad_spiky3, poly6) at d.
+void orc_lague_kernels(void* h, double d, double* out5) {
+    auto& m = ((OrcLague*)h)->s32;
+    const float f = (float)d;
+    out5[0] = m.spiky2(f);
+    out5[1] = m.spiky3(f);
+    out5[2] = m.grad_spiky2(f);
+    out5[3] = m.grad_spiky3(f);
+    out5[4] = m.poly6_w(f);
 }
 
 int orc_max_threads(void) {

@@ -46,6 +46,12 @@ def _load_oracle() -> C.CDLL:
         "orc_gauss_legendre": (None, [C.c_int, dp, dp]),
         "orc_volume_map": (C.c_int, [C.c_int, dp, up, C.c_double, C.c_double, C.c_double, fp, fp, fp, fp, C.c_int]),
         "orc_max_threads": (C.c_int, []),
+        "orc_lague_create": (vp, [vp, vp, C.c_int, C.c_int]),
+        "orc_lague_destroy": (None, [vp]),
+        "orc_lague_set_state": (None, [vp, fp, fp]),
+        "orc_lague_substeps": (None, [vp, u32]),
+        "orc_lague_get_state": (None, [vp, dp, dp, dp, dp, up]),
+        "orc_lague_kernels": (None, [vp, C.c_double, dp]),
         "orc_mesh_pseudonormals": (C.c_int, [fp, u32, up, u32, fp, fp, fp]),
         "orc_point_triangle": (C.c_int, [dp, dp, u32, dp, dp, C.POINTER(C.c_int)]),
         "orc_mesh_signed_distance": (C.c_int, [fp, u32, up, u32, fp, u32, dp, C.c_int]),
@@ -159,6 +165,42 @@ def oracle_volume_map(kind, shape_params, resolution, support_radius, tolerance=
     assert rc == 0
     return BoundaryGrid(sdf, vm, tuple(int(r) for r in res), tuple(float(v) for v in bs),
                         tuple(float(v) for v in bpos))
+
+
+class OracleLague:
+    """The oracle's LagueModel restatement (FP32, or the FP64 twin)."""
+
+    def __init__(self, params, lague, fp64=False, nthreads=None):
+        self.n = int(params.n_particles)
+        self._p, self._l = params, lague
+        self.h = ORC.orc_lague_create(C.byref(params), C.byref(lague), int(fp64), nthreads or threads())
+        assert self.h
+
+    def set_state(self, x, v=None):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1)
+        v = np.zeros_like(x) if v is None else np.ascontiguousarray(v, np.float32).reshape(-1)
+        ORC.orc_lague_set_state(self.h, fp(x), fp(v))
+
+    def substeps(self, k):
+        ORC.orc_lague_substeps(self.h, k)
+
+    def get_state(self):
+        n = self.n
+        x, v = np.empty(3 * n), np.empty(3 * n)
+        rho, near = np.empty(n), np.empty(n)
+        pid = np.empty(n, np.uint32)
+        ORC.orc_lague_get_state(self.h, dp(x), dp(v), dp(rho), dp(near), up(pid))
+        return x.reshape(n, 3), v.reshape(n, 3), rho, near, pid
+
+    def kernels(self, d):
+        out = np.empty(5)
+        ORC.orc_lague_kernels(self.h, float(d), dp(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ORC.orc_lague_destroy(self.h)
+            self.h = None
 
 
 def _mesh_arrays(vertices, triangles):

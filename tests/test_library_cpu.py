@@ -74,6 +74,14 @@ def test_scene_loader_errors(name, code, needle):
     assert out.value is None
 
 
+def test_trajectory_argument_checks(tmp_path):
+    out = C.c_void_p()
+    rc, msg = _err(V.lib.vfs_trajectory_open, None, str(tmp_path / "t").encode(), 1, C.byref(out))
+    assert rc == -1 and "null" in msg
+    assert V.lib.vfs_trajectory_record(None, 0, 0.0) == -1
+    assert V.lib.vfs_trajectory_close(None) == 0
+
+
 def test_cpp_facade_example_links():
     """The C++ facade example links against libvfs.so and starts (usage path,
     no device work)."""

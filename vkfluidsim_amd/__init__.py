@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (BoundaryObject, DeviceArray, Params, Tuning, VfsError, VFS_FLAG_NO_GRAPH, VFS_FLAG_STORE_ACCEL,
+from ._lib import (BoundaryObject, DeviceArray, LagueParams, Params, Tuning, VfsError, VFS_FLAG_NO_GRAPH, VFS_FLAG_STORE_ACCEL,
                    VFS_FLAG_TRACK_IDS, VFS_NEIGHBOUR_CAPACITY, VFS_SHAPE_BOX, VFS_SHAPE_BOX_WALLS, VFS_SHAPE_SPHERE, call, fptr, lib, uptr)
 
 __all__ = ["Sim", "Batch", "Adjoint", "Params", "Tuning", "BoundaryGrid", "SpatialHash", "VfsError", "make_params", "volume_map",
@@ -101,6 +101,71 @@ def boundary_grids(shapes, support_radius: float) -> list:
             m.close()
         else:
             out.append(volume_map(kind, sp, res, support_radius))
+    return out
+
+
+def make_lague_params(wall_damping_factor: float = 0.95, pressure_multiplier: float = 288.0,
+                      near_pressure_multiplier: float = 2.16, viscosity: float = 0.03) -> LagueParams:
+    """vfs_lague_params; the defaults are LagueModel's (lague_model.cpp:26-34)."""
+    return LagueParams(wall_damping_factor, pressure_multiplier, near_pressure_multiplier, viscosity)
+
+
+VFS_TRAJ_POSITIONS, VFS_TRAJ_VELOCITIES, VFS_TRAJ_DENSITIES, VFS_TRAJ_IDS = 1, 2, 4, 8
+_TRAJ_HEADER = np.dtype([("magic", "S8"), ("version", "<u4"), ("n_particles", "<u4"), ("fields", "<u4"),
+                         ("n_objects", "<u4"), ("n_frames", "<u8"), ("box_pos", "<f4", 3),
+                         ("box_size", "<f4", 3), ("reserved", "<u4", 2)])
+
+
+class Trajectory:
+    """Recorder of particle-state frames into a trajectory file
+    (vfs_trajectory_*); read files back with read_trajectory."""
+
+    def __init__(self, sim, path: str, fields: int = VFS_TRAJ_POSITIONS | VFS_TRAJ_VELOCITIES):
+        h = C.c_void_p()
+        call("vfs_trajectory_open", sim.handle if hasattr(sim, "handle") else sim, str(path).encode(), int(fields),
+             C.byref(h))
+        self.handle = h.value
+
+    def record(self, substep: int, time: float) -> None:
+        call("vfs_trajectory_record", self.handle, int(substep), float(time))
+
+    def close(self) -> None:
+        if self.handle:
+            call("vfs_trajectory_close", self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def read_trajectory(path: str) -> dict:
+    """A trajectory file as numpy arrays: header fields, objects (k, 16),
+    substep (f,), time (f,) and per field (f, n, 3) or (f, n) arrays."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    hdr = raw[:64].view(_TRAJ_HEADER)[0]
+    if hdr["magic"] != b"VFSTRAJ1" or hdr["version"] != 1:
+        raise ValueError(f"{path}: not a version-1 trajectory file")
+    n, fields, k, f = int(hdr["n_particles"]), int(hdr["fields"]), int(hdr["n_objects"]), int(hdr["n_frames"])
+    off = 64
+    out = {"n_particles": n, "fields": fields, "box_pos": tuple(hdr["box_pos"]), "box_size": tuple(hdr["box_size"]),
+           "objects": raw[off:off + 64 * k].view("<f4").reshape(k, 16)}
+    off += 64 * k
+    parts = [("substep", "<u8", 1), ("time", "<f8", 1)]
+    for bit, name, dt, w in [(1, "positions", "<f4", 3), (2, "velocities", "<f4", 3), (4, "densities", "<f4", 1),
+                             (8, "particle_ids", "<u4", 1)]:
+        if fields & bit:
+            parts.append((name, dt, w * n))
+    frame = np.dtype([(name, dt, (cnt,)) for name, dt, cnt in parts])
+    frames = raw[off:off + frame.itemsize * f].view(frame)
+    out["substep"] = frames["substep"][:, 0].copy()
+    out["time"] = frames["time"][:, 0].copy()
+    for bit, name, w in [(1, "positions", 3), (2, "velocities", 3), (4, "densities", 1), (8, "particle_ids", 1)]:
+        if fields & bit:
+            a = frames[name].copy()
+            out[name] = a.reshape(f, n, 3) if w == 3 else a.reshape(f, n)
     return out
 
 
@@ -374,6 +439,27 @@ def reorder_f32x3(perm: np.ndarray, src: np.ndarray) -> np.ndarray:
     call("vfs_reorder_f32x3", dp.ptr, ds.ptr, dd.ptr, n, None)
     call("vfs_stream_synchronize", None)
     return dd.to_host(np.float32, 3 * n).reshape(n, 3)
+
+
+class Lague(Sim):
+    """LagueModel (Clavet double-density) through the C ABI (vfs_lague_*); the
+    vfs_sim methods of Sim apply."""
+
+    def __init__(self, params: Params, lague: LagueParams | None = None):
+        self.params = params
+        self.n = int(params.n_particles)
+        self._objects = []
+        h = C.c_void_p()
+        call("vfs_lague_create", C.byref(params), C.byref(lague) if lague is not None else None, C.byref(h))
+        self.handle = h.value
+
+    def set_lague_params(self, lague: LagueParams) -> None:
+        call("vfs_lague_set_params", self.handle, C.byref(lague))
+
+    def get_lague_params(self) -> LagueParams:
+        lp = LagueParams()
+        call("vfs_lague_get_params", self.handle, C.byref(lp))
+        return lp
 
 
 class Batch:

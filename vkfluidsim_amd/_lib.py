@@ -33,6 +33,12 @@ class VfsError(RuntimeError):
         self.code = code
 
 
+class LagueParams(C.Structure):
+    """vfs_lague_params (LagueModel::Parameters, lague_model.h:19-24)."""
+    _fields_ = [("wall_damping_factor", C.c_float), ("pressure_multiplier", C.c_float),
+                ("near_pressure_multiplier", C.c_float), ("viscosity_strenght", C.c_float)]
+
+
 class Params(C.Structure):
     """vfs_params (SimulationParameters + SPHModel::Parameters + WCSPH parameters)."""
     _fields_ = [
@@ -177,6 +183,12 @@ SIGNATURES = {
     "vfs_mesh_signed_distance": (_i32, [_vp, _fp, _u32, C.POINTER(C.c_double)]),
     "vfs_mesh_volume_map_generate": (_i32, [_vp, _up, C.c_double, C.c_double, C.c_double, _fp, _fp, _fp, _fp]),
     "vfs_mesh_destroy": (_i32, [_vp]),
+    "vfs_trajectory_open": (_i32, [_vp, C.c_char_p, _u32, C.POINTER(_vp)]),
+    "vfs_trajectory_record": (_i32, [_vp, C.c_uint64, C.c_double]),
+    "vfs_trajectory_close": (_i32, [_vp]),
+    "vfs_lague_create": (_i32, [C.POINTER(Params), C.POINTER(LagueParams), C.POINTER(_vp)]),
+    "vfs_lague_set_params": (_i32, [_vp, C.POINTER(LagueParams)]),
+    "vfs_lague_get_params": (_i32, [_vp, C.POINTER(LagueParams)]),
     "vfs_scene_load": (_i32, [C.c_char_p, C.c_char_p, C.POINTER(_vp)]),
     "vfs_scene_sim": (_i32, [_vp, C.POINTER(_vp)]),
     "vfs_scene_reset": (_i32, [_vp]),

@@ -190,6 +190,34 @@ def obstacle_1m(spacing: float = 0.006, map_res: int = 128, side=(128, 64, 128),
                     shapes)
 
 
+def compact_block(counts, density: float, pos=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """SpawnCompactParticlesInBox (model.cpp:36-63) for the box a scene sizes
+    from `counts` (dam_break_scene.cpp:42-50): spacing cbrt(1 / density) in
+    float32, x-major with z innermost, at the box position."""
+    step = np.float32(np.cbrt(np.float32(1.0) / np.float32(density)))
+    size = step * (np.asarray(counts, np.float32) + np.float32(1.0))
+    n_axis = [int(np.round(size[a] / step)) - 1 for a in range(3)]
+    i, j, k = np.meshgrid(*[np.arange(m, dtype=np.float32) for m in n_axis], indexing="ij")
+    pts = np.stack([step * i.ravel(), step * j.ravel(), step * k.ravel()], 1) + np.asarray(pos, np.float32)
+    return pts[: int(np.prod(counts))].astype(np.float32)
+
+
+def lague_dam_break(counts=(50, 60, 50), steps: int = 100) -> Workload:
+    """The reference's Lague dam break (dam_break_scene.cpp:8-50): a 50 x 60 x 50
+    block (150,000 particles) at target density 630 in a 23 x 10 x 10 box,
+    dt = 1/120 over 3 iterations, smooth radius 0.2, the block centred in z."""
+    h, rho0 = 0.2, 630.0
+    box = (23.0, 10.0, 10.0)
+    step = np.float32(np.cbrt(np.float32(1.0) / np.float32(rho0)))
+    size_z = step * np.float32(counts[2] + 1)
+    pos = (0.0, 0.0, float(np.float32(0.0) + (np.float32(box[2]) - size_z) / np.float32(2.0)))
+    x = compact_block(counts, rho0, pos)
+    v = np.zeros_like(x)
+    n = x.shape[0]
+    return Workload("lague_dam_break", n, h, 1.0 / 120.0, steps, box, (0.0, 0.0, 0.0), x, v, [],
+                    {"rho0": rho0, "iterations": 3})
+
+
 def uniform_stress(n: int = 8_388_608, h: float = 0.02, map_res: int = 256, steps: int = 200) -> Workload:
     """n particles uniform in a 4 x 2 x 2 box with N(0, 0.5) velocities, box walls."""
     seed = SEED_BASE + 2
@@ -231,6 +259,7 @@ def batched_dam_breaks(n_scenes: int = 256, seed: int = SEED_BASE + 3, map_res: 
 WORKLOADS = {
     "dam_break_32k": lambda: dam_break(),
     "obstacle_1m": lambda: obstacle_1m(),
+    "lague_dam_break": lambda: lague_dam_break(),
     "obstacle_1m_wall_map": lambda: obstacle_1m(walls=True),
     "uniform_8m": lambda: uniform_stress(),
     "dam_break_65k": lambda: dam_break(side=(64, 32, 32), seed=SEED_BASE + 4, steps=200,

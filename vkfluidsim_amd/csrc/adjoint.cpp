@@ -108,6 +108,7 @@ int vfs_adjoint_create(vfs_sim* sim, uint32_t n_substeps, uint32_t checkpoint_ev
     *out = nullptr;
     VFS_CHECK(sim && grid, VFS_ERR_INVALID, "null argument");
     VFS_CHECK(!sim->batch, VFS_ERR_INVALID, "the adjoint runs on a single-scene simulation");
+    VFS_CHECK(sim->model == 0, VFS_ERR_INVALID, "the adjoint differentiates the WCSPH model");
     VFS_CHECK(n_substeps >= 1 && checkpoint_every >= 1, VFS_ERR_INVALID, "substeps and checkpoint interval >= 1");
     VFS_CHECK(sim->tuning.cell_scale == 1.0f, VFS_ERR_INVALID, "the adjoint needs the scene's own cell size");
     VFS_CHECK(sim->n >= 1, VFS_ERR_INVALID, "the adjoint needs at least one particle");

@@ -83,6 +83,42 @@ SimConsts make_consts(const vfs_params& p) {
     return c;
 }
 
+LagueConsts make_lague_consts(const vfs_params& p, const vfs_lague_params& lp) {
+    LagueConsts l{};
+    l.gx = p.gravity[0];
+    l.gy = p.gravity[1];
+    l.gz = p.gravity[2];
+    l.h = p.smooth_radius;
+    l.h2 = l.h * l.h;
+    l.dt = p.time_scale * p.fixed_dt / (float)p.iterations;
+    l.pf = 1.0f / 120.0f;
+    l.rho0 = p.target_density;
+    l.pm = lp.pressure_multiplier;
+    l.npm = lp.near_pressure_multiplier;
+    l.visc = lp.viscosity_strenght;
+    l.damp = lp.wall_damping_factor;
+    // CalcKernelCoefficients (model.cpp:224-233): std::pow(float, int) is computed in double
+    const float pi_f = 3.14159265358979323846f;
+    const float r = p.smooth_radius;
+    l.sp3 = 15.0f / (pi_f * (float)std::pow((double)r, 6));
+    l.sp2 = 15.0f / (2.0f * pi_f * (float)std::pow((double)r, 5));
+    l.dsp3 = 45.0f / (pi_f * (float)std::pow((double)r, 6));
+    l.dsp2 = 15.0f / (pi_f * (float)std::pow((double)r, 5));
+    // Poly6 scale (kernels_3d.slang:19) with the shader's PI = 3.1415926 and
+    // the integral power r^9 by binary powering in double, rounded once
+    double base = (double)std::fabs(r), r9 = 1.0;
+    for (unsigned e = 9; e; e >>= 1) {
+        if (e & 1u) r9 *= base;
+        base *= base;
+    }
+    l.poly6 = 315.0f / ((64.0f * 3.1415926f) * (float)r9);
+    for (int a = 0; a < 3; a++) {
+        l.bb_lo[a] = p.bbox_pos[a];
+        l.bb_hi[a] = p.bbox_pos[a] + p.bbox_size[a];
+    }
+    return l;
+}
+
 }  // namespace vfs
 
 using namespace vfs;
@@ -97,7 +133,7 @@ void free_sim(vfs_sim* s) {
     void* ptrs[] = {s->x,           s->v,       s->xs,         s->vs,       s->bnd,      s->accel,
                     s->rho,         s->pid,     s->pids,       s->keys,     s->sorted_keys, s->perm,
                     s->offsets,     s->cell_start, s->sort_ws, s->objs_dev, s->nbr,      s->nbr_count,
-                    s->consts_dev,  s->wave_scene};
+                    s->consts_dev,  s->wave_scene, s->pred, s->ps, s->near_rho};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (float* g : s->grids)
@@ -350,6 +386,7 @@ int set_params_impl(vfs_sim* sim, uint32_t scene, const vfs_params* params) {
     sc.cell0 = c.cell;
     c.cell = sc.cell0 * sim->tuning.cell_scale;
     sc.consts = c;
+    if (sim->model == 1) sim->lague = make_lague_consts(*params, sim->lague_params);
     if (sc.consts.cell != old_cell) sim->keys_valid = false;
     sim->drop_graph();
     return sim->upload_consts();
@@ -641,6 +678,55 @@ int vfs_sim_create(const vfs_params* params, const vfs_boundary_object* objects,
                    vfs_sim** out) {
     VFS_CHECK(n_objects == 0 || objects, VFS_ERR_INVALID, "objects is null");
     return create_impl(params, 1, objects, &n_objects, false, out);
+}
+
+int vfs_lague_create(const vfs_params* params, const vfs_lague_params* lague, vfs_sim** out) {
+    VFS_CHECK(out, VFS_ERR_INVALID, "out is null");
+    *out = nullptr;
+    // LagueModel's default parameters (lague_model.cpp:26-34)
+    const vfs_lague_params lp = lague ? *lague : vfs_lague_params{0.95f, 288.0f, 2.16f, 0.03f};
+    uint32_t none = 0;
+    vfs_sim* s = nullptr;
+    VFS_TRY(create_impl(params, 1, nullptr, &none, false, &s));
+    s->model = 1;
+    s->lague_params = lp;
+    s->lague = make_lague_consts(s->scenes[0].params, lp);
+    const uint32_t n = s->n;
+    int r = 0;
+    r |= dev_alloc(&s->pred, n);
+    // the cell walk reads candidates in groups of kCandidateGroup past the last one
+    r |= dev_alloc(&s->ps, (size_t)n + kCandidateGroup);
+    r |= dev_alloc(&s->near_rho, n);
+    if (r) {
+        free_sim(s);
+        return VFS_ERR_NOMEM;
+    }
+    if (n && (hipMemsetAsync(s->near_rho, 0, (size_t)n * 4, s->stream) != hipSuccess ||
+              hipMemsetAsync(s->pred, 0, (size_t)n * 16, s->stream) != hipSuccess ||
+              hipMemsetAsync(s->ps, 0, ((size_t)n + kCandidateGroup) * 16, s->stream) != hipSuccess ||
+              hipStreamSynchronize(s->stream) != hipSuccess)) {
+        free_sim(s);
+        return fail(VFS_ERR_HIP, "state clear failed");
+    }
+    *out = s;
+    return 0;
+}
+
+int vfs_lague_set_params(vfs_sim* sim, const vfs_lague_params* lague) {
+    VFS_CHECK(sim && lague, VFS_ERR_INVALID, "null argument");
+    VFS_CHECK(sim->model == 1, VFS_ERR_INVALID, "not a Lague model simulation");
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    sim->lague_params = *lague;
+    sim->lague = make_lague_consts(sim->scenes[0].params, *lague);
+    sim->drop_graph();
+    return 0;
+}
+
+int vfs_lague_get_params(const vfs_sim* sim, vfs_lague_params* lague) {
+    VFS_CHECK(sim && lague, VFS_ERR_INVALID, "null argument");
+    VFS_CHECK(sim->model == 1, VFS_ERR_INVALID, "not a Lague model simulation");
+    *lague = sim->lague_params;
+    return 0;
 }
 
 int vfs_sim_destroy(vfs_sim* sim) {

@@ -45,6 +45,14 @@ struct vfs_sim {
     DevObject* objs_dev = nullptr;
     std::vector<float*> grids;
 
+    // model: 0 = WCSPHWithBoundaryModel, 1 = LagueModel (predictions and near
+    // densities in pred / ps / near_rho)
+    int model = 0;
+    LagueConsts lague{};
+    vfs_lague_params lague_params{};
+    float4 *pred = nullptr, *ps = nullptr;
+    float* near_rho = nullptr;
+
     bool keys_valid = false;
     bool stepped = false;
     hipGraph_t graph = nullptr;
@@ -71,6 +79,9 @@ struct vfs_sim {
         b.accel = accel;
         b.nbr = nbr;
         b.nbr_count = nbr_count;
+        b.pred = pred;
+        b.ps = ps;
+        b.near_rho = near_rho;
         b.n = n;
         return b;
     }
@@ -101,7 +112,31 @@ struct vfs_sim {
         return 0;
     }
 
+    // LagueModel::Step body (lague_model.cpp:106-122) for one substep.
+    int record_lague_substep(hipStream_t s, StageTimer* timer) {
+        const StepBuffers b = buffers();
+        const SimConsts& c = scenes[0].consts;
+        if (timer) timer->begin(s, "predict");
+        VFS_TRY(launch_lague_predict(b, c, lague, s));
+        if (timer) timer->end(s);
+        VFS_TRY(sort_keys_iota(plan, keys, sorted_keys, perm, sort_ws, s, timer, offsets, (size_t)n + 1, n));
+        if (timer) timer->begin(s, "reorder");
+        VFS_TRY(launch_reorder_state(b, s));
+        if (timer) timer->end(s);
+        if (timer) timer->begin(s, "cell_ranges");
+        VFS_TRY(launch_cell_start_scan(plan, offsets, cell_start, sort_ws, s));
+        if (timer) timer->end(s);
+        if (timer) timer->begin(s, "density");
+        VFS_TRY(launch_lague_density(b, c, lague, s));
+        if (timer) timer->end(s);
+        if (timer) timer->begin(s, "forces_integrate");
+        VFS_TRY(launch_lague_forces(b, c, lague, s));
+        if (timer) timer->end(s);
+        return 0;
+    }
+
     int record_substep(hipStream_t s, StageTimer* timer) {
+        if (model == 1) return record_lague_substep(s, timer);
         const StepBuffers b = buffers();
         const SceneTable table{consts_dev, wave_scene};
         const SceneTable* tp = batch ? &table : nullptr;
@@ -123,7 +158,7 @@ struct vfs_sim {
     }
 
     int ensure_keys(hipStream_t s) {
-        if (keys_valid) return 0;
+        if (keys_valid || model == 1) return 0;  // the Lague substep hashes its own predictions
         for (const auto& sc : scenes)
             VFS_TRY(launch_hash_keys(reinterpret_cast<const float*>(x + sc.base), 4, sc.n, sc.consts.cell, sc.n,
                                      keys + sc.base, s, sc.base));

@@ -83,6 +83,11 @@ struct StepBuffers {
     uint32_t* nbr;       // nbr_cap * n: neighbour lists of the substep in chunks of four,
                          // chunk-major (uint4 chunk g of particle i at g * n + i)
     uint32_t* nbr_count; // n: list lengths (> nbr_cap: overflowed)
+    // Lague model only (null otherwise): predicted positions in the current
+    // order, their sorted copy, near densities
+    float4* pred;
+    float4* ps;
+    float* near_rho;
     uint32_t n;          // entries of every per-particle array (all scenes of a batch, padded)
 };
 
@@ -91,6 +96,11 @@ struct StepBuffers {
 constexpr uint32_t kCandidateGroup = 4;
 
 int launch_reorder_state(const StepBuffers& b, hipStream_t s);
+// Lague model stages (one scene): predictions and their hash keys, density
+// and near density, pressure + viscosity + integration + collisions.
+int launch_lague_predict(const StepBuffers& b, const SimConsts& c, const LagueConsts& lc, hipStream_t s);
+int launch_lague_density(const StepBuffers& b, const SimConsts& c, const LagueConsts& lc, hipStream_t s);
+int launch_lague_forces(const StepBuffers& b, const SimConsts& c, const LagueConsts& lc, hipStream_t s);
 // block: 128 or 256 threads; staged: stage stencil rows of positions in LDS.
 // table == nullptr: one scene with constants c over all b.n entries; else a
 // batch whose waves read their scene's constants from the table (c unused).

@@ -101,6 +101,22 @@ struct SimConsts {
     uint32_t obj_begin, obj_count;  // its boundary objects in the object array
 };
 
+// Constants of the Lague (Clavet double-density) model (lague_model.slang,
+// kernels_3d.slang:16-60, CalcKernelCoefficients model.cpp:224-233).
+struct LagueConsts {
+    float gx, gy, gz;
+    float h, h2;
+    float dt;
+    float pf;              // prediction factor 1/120
+    float rho0;            // target density
+    float pm, npm;         // pressure multipliers
+    float visc, damp;      // viscosity, wall damping
+    float sp2, sp3;        // SpikyPow2 / SpikyPow3 scales
+    float dsp2, dsp3;      // their derivative scales
+    float poly6;           // Poly6 scale
+    float bb_lo[3], bb_hi[3];
+};
+
 // Per-wave scene lookup of batched launches: a batch pads every scene to a
 // multiple of 64 entries so that a wave never spans two scenes.
 struct SceneTable {
@@ -129,5 +145,6 @@ struct DevObject {
 // float expressions (model.cpp:225-233, wcsph_with_boundary_model.cpp:84-89,
 // wcsph_with_boundary_model.slang:39-41).
 SimConsts make_consts(const vfs_params& p);
+LagueConsts make_lague_consts(const vfs_params& p, const vfs_lague_params& lp);
 
 }  // namespace vfs

@@ -150,6 +150,7 @@ __global__ __launch_bounds__(256) void k_reorder_state(StepBuffers b, uint32_t n
     b.xs[i] = b.x[s];
     b.vs[i] = b.v[s];
     if (b.pid) b.pids[i] = b.pid[s];
+    if (b.ps) b.ps[i] = b.pred[s];
     const uint32_t k = b.sorted_keys[i];
     if (i == 0 || b.sorted_keys[i - 1] != k) b.offsets[k] = i;
 }
@@ -421,6 +422,158 @@ __global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimCo
                                  c.fm);
 }
 
+// ---------------------------------------------------------- Lague model
+// SpikyPow2 / SpikyPow3 / their gradients / Poly6 (kernels_3d.slang:16-60).
+__device__ __forceinline__ float spiky2(float d, const LagueConsts& l) {
+    const float v = l.h - d;
+    return d < l.h ? (v * v) * l.sp2 : 0.0f;
+}
+__device__ __forceinline__ float spiky3(float d, const LagueConsts& l) {
+    const float v = l.h - d;
+    return d < l.h ? ((v * v) * v) * l.sp3 : 0.0f;
+}
+__device__ __forceinline__ float grad_spiky2(float d, const LagueConsts& l) {
+    const float v = l.h - d;
+    return d <= l.h ? (-v) * l.dsp2 : 0.0f;
+}
+__device__ __forceinline__ float grad_spiky3(float d, const LagueConsts& l) {
+    const float v = l.h - d;
+    return d <= l.h ? ((-v) * v) * l.dsp3 : 0.0f;
+}
+__device__ __forceinline__ float poly6(float d, const LagueConsts& l) {
+    const float w = l.h * l.h - d * d;
+    return d < l.h ? ((w * w) * w) * l.poly6 : 0.0f;
+}
+
+// ExternalForces (lague_model.slang:211-221) and the hash key of the
+// prediction, which the spatial hash of this substep sorts (model.cpp:210-222).
+__global__ __launch_bounds__(256) void k_lague_predict(StepBuffers b, SimConsts c, LagueConsts l, uint32_t nblocks) {
+    const uint32_t i = xcd_block(nblocks) * 256u + threadIdx.x;
+    if (i >= b.n) return;
+    const float4 x = b.x[i];
+    const float4 v0 = b.v[i];
+    const float vx = v0.x + l.gx * l.dt, vy = v0.y + l.gy * l.dt, vz = v0.z + l.gz * l.dt;
+    const float px = x.x + vx * l.pf, py = x.y + vy * l.pf, pz = x.z + vz * l.pf;
+    b.v[i] = make_float4(vx, vy, vz, 0.0f);
+    b.pred[i] = make_float4(px, py, pz, 0.0f);
+    b.keys[i] = c.base + fastmod(hash_cell(floor_to_cell(px / c.cell), floor_to_cell(py / c.cell),
+                                           floor_to_cell(pz / c.cell)),
+                                 c.fm);
+}
+
+// Walks the 27-cell stencil of the sorted predictions in the reference's
+// order, calling hit(j, q, valid) for every candidate.
+template <class Hit>
+__device__ __forceinline__ void lague_walk(const float4& p, bool active, const StepBuffers& b, const SimConsts& c,
+                                           Hit&& hit) {
+    const int32_t cx = floor_to_cell(p.x / c.cell), cy = floor_to_cell(p.y / c.cell),
+                  cz = floor_to_cell(p.z / c.cell);
+    const float4* __restrict__ ps = b.ps;
+    const int cnt = 0;
+    auto none = [] {};
+    auto load = [&](uint32_t j0, int u) { return (ps + j0)[u]; };
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) {
+            if (!active) continue;
+            const RowRanges r = row_ranges(cx, cy, cz, dy, dz, c, b.cell_start);
+            walk_range(r.b0, r.e0, load, cnt, hit, none);
+            if (r.nr == 3) {
+                walk_range(r.b1, r.e1, load, cnt, hit, none);
+                walk_range(r.b2, r.e2, load, cnt, hit, none);
+            }
+        }
+}
+
+// CalculateDensities (lague_model.slang:60-97, 222-230): density and near
+// density of the predicted positions, self included.
+__global__ __launch_bounds__(256) void k_lague_density(StepBuffers b, SimConsts c, LagueConsts l, uint32_t nblocks) {
+    const uint32_t i = xcd_block(nblocks) * 256u + threadIdx.x;
+    const bool active = i < b.n;
+    const float4 p = b.ps[active ? i : 0u];
+    float density = 0.0f, near = 0.0f;
+    lague_walk(p, active, b, c, [&](uint32_t, const float4& q, bool valid) {
+        const float ox = q.x - p.x, oy = q.y - p.y, oz = q.z - p.z;
+        const float r2 = dot3(ox, oy, oz, ox, oy, oz);
+        if (valid && !(r2 > l.h2)) {
+            const float d = sqrtf(r2);
+            density += spiky2(d, l);
+            near += spiky3(d, l);
+        }
+    });
+    if (!active) return;
+    b.rho[i] = density;
+    b.near_rho[i] = near;
+}
+
+// CalculatePressureForces (lague_model.slang:99-160, 231-238) with the
+// velocities from before this pass, then UpdatePositions with
+// ResolveCollisions (slang:162-200, 240-250).
+__global__ __launch_bounds__(256) void k_lague_forces(StepBuffers b, SimConsts c, LagueConsts l, uint32_t nblocks) {
+    const uint32_t i = xcd_block(nblocks) * 256u + threadIdx.x;
+    const bool active = i < b.n;
+    const uint32_t ii = active ? i : 0u;
+    const float4 p = b.ps[ii];
+    const float4 vel = b.vs[ii];
+    const float density = b.rho[ii], density_near = b.near_rho[ii];
+    const float pressure = (density - l.rho0) * l.pm;
+    const float near_pressure = l.npm * density_near;
+    float pfx = 0.0f, pfy = 0.0f, pfz = 0.0f, vfx = 0.0f, vfy = 0.0f, vfz = 0.0f;
+    lague_walk(p, active, b, c, [&](uint32_t j, const float4& q, bool valid) {
+        const float ox = q.x - p.x, oy = q.y - p.y, oz = q.z - p.z;
+        const float r2 = dot3(ox, oy, oz, ox, oy, oz);
+        if (!valid || j == i || r2 > l.h2) return;
+        const float nd = b.rho[j], nnd = b.near_rho[j];
+        const float np = (nd - l.rho0) * l.pm;
+        const float nnp = l.npm * nnd;
+        const float shared = (pressure + np) * 0.5f;
+        const float shared_near = (near_pressure + nnp) * 0.5f;
+        const float d = sqrtf(r2);
+        float dx = 0.0f, dy = 1.0f, dz = 0.0f;
+        if (d > 0.0f) {
+            const float inv = 1.0f / d;
+            dx = ox * inv;
+            dy = oy * inv;
+            dz = oz * inv;
+        }
+        const float dd = grad_spiky2(d, l), ndd = grad_spiky3(d, l);
+        const float inv_nd = 1.0f / nd, inv_nnd = 1.0f / nnd;
+        pfx += ((dx * dd) * shared) * inv_nd;
+        pfy += ((dy * dd) * shared) * inv_nd;
+        pfz += ((dz * dd) * shared) * inv_nd;
+        pfx += ((dx * ndd) * shared_near) * inv_nnd;
+        pfy += ((dy * ndd) * shared_near) * inv_nnd;
+        pfz += ((dz * ndd) * shared_near) * inv_nnd;
+        const float4 vj = b.vs[j];
+        const float w = poly6(d, l);
+        vfx += (vj.x - vel.x) * w;
+        vfy += (vj.y - vel.y) * w;
+        vfz += (vj.z - vel.z) * w;
+    });
+    if (!active) return;
+    const float inv_density = 1.0f / density;
+    const float ax = pfx * inv_density + vfx * l.visc;
+    const float ay = pfy * inv_density + vfy * l.visc;
+    const float az = pfz * inv_density + vfz * l.visc;
+    float v3[3] = {vel.x + ax * l.dt, vel.y + ay * l.dt, vel.z + az * l.dt};
+    const float4 x4 = b.xs[i];
+    float x3[3] = {x4.x + v3[0] * l.dt, x4.y + v3[1] * l.dt, x4.z + v3[2] * l.dt};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        if (x3[a] < l.bb_lo[a]) {
+            x3[a] = l.bb_lo[a];
+            v3[a] = (-v3[a]) * l.damp;
+        }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+        if (x3[a] > l.bb_hi[a]) {
+            x3[a] = l.bb_hi[a];
+            v3[a] = (-v3[a]) * l.damp;
+        }
+    b.x[i] = make_float4(x3[0], x3[1], x3[2], 0.0f);
+    b.v[i] = make_float4(v3[0], v3[1], v3[2], 0.0f);
+    if (b.pid) b.pid[i] = b.pids[i];
+}
+
 // ---------------------------------------------------------- volume maps
 struct ShapeArgs {
     int kind;
@@ -515,6 +668,27 @@ int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, const Scen
             hipLaunchKernelGGL((k_forces_integrate<256, false>), dim3(nb), dim3(256), 0, s, b, c, t, nb);
     }
     return check_launch("forces_integrate");
+}
+
+int launch_lague_predict(const StepBuffers& b, const SimConsts& c, const LagueConsts& lc, hipStream_t s) {
+    if (b.n == 0) return 0;
+    const uint32_t nb = particle_blocks(b.n, 256);
+    hipLaunchKernelGGL(k_lague_predict, dim3(nb), dim3(256), 0, s, b, c, lc, nb);
+    return check_launch("lague_predict");
+}
+
+int launch_lague_density(const StepBuffers& b, const SimConsts& c, const LagueConsts& lc, hipStream_t s) {
+    if (b.n == 0) return 0;
+    const uint32_t nb = particle_blocks(b.n, 256);
+    hipLaunchKernelGGL(k_lague_density, dim3(nb), dim3(256), 0, s, b, c, lc, nb);
+    return check_launch("lague_density");
+}
+
+int launch_lague_forces(const StepBuffers& b, const SimConsts& c, const LagueConsts& lc, hipStream_t s) {
+    if (b.n == 0) return 0;
+    const uint32_t nb = particle_blocks(b.n, 256);
+    hipLaunchKernelGGL(k_lague_forces, dim3(nb), dim3(256), 0, s, b, c, lc, nb);
+    return check_launch("lague_forces");
 }
 
 int launch_volume_map(int kind, const double* shape_params, const uint32_t* res, double support_radius,
