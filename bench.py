@@ -308,6 +308,54 @@ def bench_adjoint(steps: int, every: int, repeats: int = 3) -> dict:
     }
 
 
+def bench_lague(steps: int, cpu_steps: int, with_cpu: bool) -> dict:
+    """The reference's Lague dam break (dam_break_scene.cpp:8-50): 150,000
+    particles, target density 630, 23 x 10 x 10 box, smooth radius 0.2, three
+    substeps of 1/360 per frame.  Particle-substeps/s of the graph-replayed
+    substep, per-stage times, and the FP32 oracle (OpenMP) on `cpu_steps`
+    substeps of the same state."""
+    import vkfluidsim_amd as V
+    from vkfluidsim_amd import configs as CF
+
+    w = CF.lague_dam_break()
+    p = V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos, rho0=w.params_kw["rho0"],
+                      iterations=w.params_kw["iterations"])
+    sim = V.Lague(p)
+    sim.set_state(w.x, w.v)
+    sim.substeps(20)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.substeps(steps)
+    sim.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    stages = sim.profile(3)
+    x, v, rho, _ = sim.get_state()
+    out = {
+        "workload": f"Lague dam break, {w.n} particles (50 x 60 x 50 at density 630), substeps 20-{20 + steps}",
+        "particle_steps_per_s": w.n / dt,
+        "ms_per_substep": dt * 1e3,
+        "substeps": steps,
+        "stages_ms": {name: round(ms, 4) for name, ms, _ in stages},
+        "end_state": {"finite": bool(np.isfinite(x).all() and np.isfinite(v).all()),
+                      "rho_mean": float(rho.mean()), "max_speed": float(np.linalg.norm(v, axis=1).max())},
+    }
+    sim.close()
+    if with_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import harness as H
+
+        o = H.OracleLague(p, V.make_lague_params(), nthreads=H.threads())
+        o.set_state(w.x, w.v)
+        o.substeps(1)
+        t0 = time.perf_counter()
+        o.substeps(cpu_steps)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": w.n * cpu_steps / tc, "unit": "particle-steps/s", "cores": H.threads(),
+                               "kind": "port", "sample": f"{cpu_steps} substeps after the first (FP32 oracle, "
+                                                         f"OpenMP, {tc:.1f} s)"}
+    return out
+
+
 def summarize_stats(db: str) -> str:
     """Text table of a rocprofv3 --kernel-trace --stats database (top_kernels,
     whose durations read in microseconds; printed in milliseconds)."""
@@ -368,6 +416,8 @@ def main() -> int:
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--looped-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
+    ap.add_argument("--no-lague", action="store_true", help="skip the Lague model measurement")
+    ap.add_argument("--lague-steps", type=int, default=300, help="timed substeps of the Lague dam break")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
@@ -476,6 +526,7 @@ def main() -> int:
     out["batched"] = None if args.no_batched else bench_batched(args.batched_scenes, args.batched_steps,
                                                                   args.looped_steps)
     out["adjoint"] = None if args.no_adjoint else bench_adjoint(args.adjoint_steps, args.adjoint_every)
+    out["lague"] = None if args.no_lague else bench_lague(args.lague_steps, 100, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, grids, args.cpu_steps)
     else:
