@@ -112,3 +112,14 @@ def test_lague_fp64_drift():
     ex, ev, er = H.rel_err(x, ox), H.rel_err(v, ov), H.rel_err(rho, orho)
     print(f"Lague FP32 vs FP64 after 30 substeps: x {ex:.2e}, v {ev:.2e}, rho {er:.2e}")
     assert ex < 1e-4 and er < 1e-3 and ev < 1e-2
+
+
+def test_lague_profile_stages():
+    w = _small()
+    sim = V.Lague(_params(w))
+    sim.set_state(w.x, w.v)
+    names = [s[0] for s in sim.profile(3)]
+    assert names == ["predict", "sort_prepare", "sort_histogram", "sort_pass", "reorder", "cell_ranges", "density",
+                     "forces_integrate"]
+    with pytest.raises(V.VfsError, match="no boundary objects"):
+        sim.get_boundary()

@@ -583,6 +583,7 @@ int profile_impl(vfs_sim* sim, uint32_t n, float* ms, uint32_t* launches, const 
     }
     uint32_t m = 0;
     for (uint32_t i = 0; i < kN && m < max_stages; i++) {
+        if (cnt[i] == 0) continue;  // a stage the model does not have
         if (ms) ms[m] = (float)(tot[i] / n);
         if (launches) launches[m] = cnt[i] / n;
         if (names) names[m] = kNames[i];
@@ -777,6 +778,7 @@ int vfs_sim_get_hash(vfs_sim* sim, uint32_t* sorted_keys, uint32_t* sorted_indic
 }
 
 int vfs_sim_get_boundary(vfs_sim* sim, float* boundary_density, float* boundary_gradient) {
+    VFS_CHECK(!sim || sim->model == 0, VFS_ERR_INVALID, "the Lague model has no boundary objects");
     return get_boundary_impl(sim, 0, boundary_density, boundary_gradient);
 }
 

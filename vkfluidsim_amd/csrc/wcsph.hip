@@ -485,9 +485,12 @@ __device__ __forceinline__ void lague_walk(const float4& p, bool active, const S
 }
 
 // CalculateDensities (lague_model.slang:60-97, 222-230): density and near
-// density of the predicted positions, self included.
-__global__ __launch_bounds__(256) void k_lague_density(StepBuffers b, SimConsts c, LagueConsts l, uint32_t nblocks) {
-    const uint32_t i = xcd_block(nblocks) * 256u + threadIdx.x;
+// density of the predicted positions, self included.  The force pass reads a
+// neighbour's pressure with its prediction (ps.w) and its near pressure and
+// density reciprocals from b.bnd (unused by this model): the same values the
+// reference derives per pair.
+__global__ __launch_bounds__(128) void k_lague_density(StepBuffers b, SimConsts c, LagueConsts l, uint32_t nblocks) {
+    const uint32_t i = xcd_block(nblocks) * 128u + threadIdx.x;
     const bool active = i < b.n;
     const float4 p = b.ps[active ? i : 0u];
     float density = 0.0f, near = 0.0f;
@@ -503,30 +506,30 @@ __global__ __launch_bounds__(256) void k_lague_density(StepBuffers b, SimConsts 
     if (!active) return;
     b.rho[i] = density;
     b.near_rho[i] = near;
+    reinterpret_cast<float*>(b.ps)[4 * (size_t)i + 3] = (density - l.rho0) * l.pm;
+    b.bnd[i] = make_float4(l.npm * near, 1.0f / density, 1.0f / near, 0.0f);
 }
 
 // CalculatePressureForces (lague_model.slang:99-160, 231-238) with the
 // velocities from before this pass, then UpdatePositions with
 // ResolveCollisions (slang:162-200, 240-250).
-__global__ __launch_bounds__(256) void k_lague_forces(StepBuffers b, SimConsts c, LagueConsts l, uint32_t nblocks) {
-    const uint32_t i = xcd_block(nblocks) * 256u + threadIdx.x;
+__global__ __launch_bounds__(128) void k_lague_forces(StepBuffers b, SimConsts c, LagueConsts l, uint32_t nblocks) {
+    const uint32_t i = xcd_block(nblocks) * 128u + threadIdx.x;
     const bool active = i < b.n;
     const uint32_t ii = active ? i : 0u;
     const float4 p = b.ps[ii];
     const float4 vel = b.vs[ii];
-    const float density = b.rho[ii], density_near = b.near_rho[ii];
-    const float pressure = (density - l.rho0) * l.pm;
-    const float near_pressure = l.npm * density_near;
+    const float4 own = b.bnd[ii];
+    const float pressure = p.w;
+    const float near_pressure = own.x;
     float pfx = 0.0f, pfy = 0.0f, pfz = 0.0f, vfx = 0.0f, vfy = 0.0f, vfz = 0.0f;
     lague_walk(p, active, b, c, [&](uint32_t j, const float4& q, bool valid) {
         const float ox = q.x - p.x, oy = q.y - p.y, oz = q.z - p.z;
         const float r2 = dot3(ox, oy, oz, ox, oy, oz);
         if (!valid || j == i || r2 > l.h2) return;
-        const float nd = b.rho[j], nnd = b.near_rho[j];
-        const float np = (nd - l.rho0) * l.pm;
-        const float nnp = l.npm * nnd;
-        const float shared = (pressure + np) * 0.5f;
-        const float shared_near = (near_pressure + nnp) * 0.5f;
+        const float4 nb = b.bnd[j];  // near pressure, 1 / density, 1 / near density
+        const float shared = (pressure + q.w) * 0.5f;
+        const float shared_near = (near_pressure + nb.x) * 0.5f;
         const float d = sqrtf(r2);
         float dx = 0.0f, dy = 1.0f, dz = 0.0f;
         if (d > 0.0f) {
@@ -536,7 +539,7 @@ __global__ __launch_bounds__(256) void k_lague_forces(StepBuffers b, SimConsts c
             dz = oz * inv;
         }
         const float dd = grad_spiky2(d, l), ndd = grad_spiky3(d, l);
-        const float inv_nd = 1.0f / nd, inv_nnd = 1.0f / nnd;
+        const float inv_nd = nb.y, inv_nnd = nb.z;
         pfx += ((dx * dd) * shared) * inv_nd;
         pfy += ((dy * dd) * shared) * inv_nd;
         pfz += ((dz * dd) * shared) * inv_nd;
@@ -550,7 +553,7 @@ __global__ __launch_bounds__(256) void k_lague_forces(StepBuffers b, SimConsts c
         vfz += (vj.z - vel.z) * w;
     });
     if (!active) return;
-    const float inv_density = 1.0f / density;
+    const float inv_density = own.y;
     const float ax = pfx * inv_density + vfx * l.visc;
     const float ay = pfy * inv_density + vfy * l.visc;
     const float az = pfz * inv_density + vfz * l.visc;
@@ -679,15 +682,15 @@ int launch_lague_predict(const StepBuffers& b, const SimConsts& c, const LagueCo
 
 int launch_lague_density(const StepBuffers& b, const SimConsts& c, const LagueConsts& lc, hipStream_t s) {
     if (b.n == 0) return 0;
-    const uint32_t nb = particle_blocks(b.n, 256);
-    hipLaunchKernelGGL(k_lague_density, dim3(nb), dim3(256), 0, s, b, c, lc, nb);
+    const uint32_t nb = particle_blocks(b.n, 128);
+    hipLaunchKernelGGL(k_lague_density, dim3(nb), dim3(128), 0, s, b, c, lc, nb);
     return check_launch("lague_density");
 }
 
 int launch_lague_forces(const StepBuffers& b, const SimConsts& c, const LagueConsts& lc, hipStream_t s) {
     if (b.n == 0) return 0;
-    const uint32_t nb = particle_blocks(b.n, 256);
-    hipLaunchKernelGGL(k_lague_forces, dim3(nb), dim3(256), 0, s, b, c, lc, nb);
+    const uint32_t nb = particle_blocks(b.n, 128);
+    hipLaunchKernelGGL(k_lague_forces, dim3(nb), dim3(128), 0, s, b, c, lc, nb);
     return check_launch("lague_forces");
 }
 
