@@ -1688,6 +1688,77 @@ int orc_adjoint_run(void* h, uint32_t k, const orc::GridC* grid, const double* t
     return 0;
 }
 
+// The FP64 adjoint of the FP32 trajectory: the forward runs on the FP32
+// handle h32 (the trajectory the GPU reproduces bit for bit), its tape is
+// widened to double, and the loss gradient and the adjoint use the model of
+// the FP64 handle h64 (same parameters and objects; its state is unused).
+// Separates the adjoint's arithmetic from the divergence of FP32 and FP64
+// trajectories.
+int orc_adjoint_run_f32_trajectory(void* h32, void* h64, uint32_t k, const orc::GridC* grid, const double* target,
+                                   double* loss, double* grad_x, double* grad_v, double* grad_nu) {
+    auto* o32 = (OrcSim*)h32;
+    auto* o64 = (OrcSim*)h64;
+    if (o32->fp64 || !o64->fp64 || !grid) return -1;
+    auto& sim = o32->s32;
+    const orc::Model<double>& m = o64->s64.m;
+    const uint32_t n = sim.n;
+    std::vector<orc::Sim<float>::TapeStep> tape32;
+    sim.tape = &tape32;
+    for (uint32_t s = 0; s < k; s++) sim.substep();
+    sim.tape = nullptr;
+    sim.lists.clear();
+    auto widen = [](const std::vector<float>& a) { return std::vector<double>(a.begin(), a.end()); };
+    std::vector<orc::Sim<double>::TapeStep> tape(tape32.size());
+    for (size_t s = 0; s < tape32.size(); s++) {
+        tape[s].perm = tape32[s].perm;
+        tape[s].x = widen(tape32[s].x);
+        tape[s].v = widen(tape32[s].v);
+        tape[s].rho = widen(tape32[s].rho);
+        tape[s].psi = widen(tape32[s].psi);
+        tape[s].bgrad = widen(tape32[s].bgrad);
+        tape[s].nbr.swap(tape32[s].nbr);
+    }
+    tape32.clear();
+    const std::vector<double> x = widen(sim.x);
+    std::vector<double> rg;
+    orc::grid_density<double>(m, x, n, *grid, rg);
+    const size_t G = rg.size();
+    double L = 0.0;
+    for (size_t g = 0; g < G; g++) {
+        const double d = rg[g] - (target ? target[g] : 0.0);
+        L += d * d;
+    }
+    if (loss) *loss = L / (double)G;
+    std::vector<double> gx(3 * (size_t)n, 0.0), gv(3 * (size_t)n, 0.0);
+    for (uint32_t j = 0; j < n; j++) {
+        int lo[3], hi[3];
+        orc::grid_range<double>(*grid, &x[3 * j], m.h, lo, hi);
+        for (int iz = lo[2]; iz <= hi[2]; iz++)
+            for (int iy = lo[1]; iy <= hi[1]; iy++)
+                for (int ix = lo[0]; ix <= hi[0]; ix++) {
+                    double gp[3], d[3];
+                    orc::grid_point<double>(*grid, ix, iy, iz, gp);
+                    for (int a = 0; a < 3; a++) d[a] = x[3 * j + a] - gp[a];
+                    const double r2 = orc::dot3<double>(d, d);
+                    if (r2 > m.h2) continue;
+                    const double r = std::sqrt(r2);
+                    if (!(r > 1e-9)) continue;
+                    const size_t idx = (size_t)ix + grid->dims[0] * ((size_t)iy + (size_t)grid->dims[1] * iz);
+                    const double cg = 2.0 * (rg[idx] - (target ? target[idx] : 0.0)) / (double)G;
+                    const double s2 = cg * m.mass * (m.h * m.dW(r)) / r;
+                    for (int a = 0; a < 3; a++) gx[3 * j + a] += s2 * d[a];
+                }
+    }
+    double nu_bar = 0.0;
+    for (int s = (int)k - 1; s >= 0; s--) orc::adjoint_substep<double>(m, tape[s], n, gx, gv, nu_bar);
+    for (size_t i = 0; i < 3 * (size_t)n; i++) {
+        if (grad_x) grad_x[i] = gx[i];
+        if (grad_v) grad_v[i] = gv[i];
+    }
+    if (grad_nu) *grad_nu = nu_bar;
+    return 0;
+}
+
 void orc_sim_substeps(void* h, uint32_t k) {
     auto* s = (OrcSim*)h;
     for (uint32_t i = 0; i < k; i++) {

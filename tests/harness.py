@@ -38,6 +38,7 @@ def _load_oracle() -> C.CDLL:
         "orc_sim_set_state": (None, [vp, fp, fp, u32, u32]),
         "orc_sim_set_state_f64": (None, [vp, dp, dp, u32, u32]),
         "orc_adjoint_run": (C.c_int, [vp, u32, vp, dp, dp, dp, dp, dp, dp]),
+        "orc_adjoint_run_f32_trajectory": (C.c_int, [vp, vp, u32, vp, dp, dp, dp, dp, dp]),
         "orc_sim_substeps": (None, [vp, u32]),
         "orc_sim_get_state": (None, [vp, dp, dp, dp, up, dp]),
         "orc_sim_get_state_f32": (None, [vp, fp, fp, fp]),
@@ -294,6 +295,26 @@ def oracle_adjoint(params, grids, x0, v0, steps, origin, spacing, dims, target=N
                              dp(gv), dp(gnu), dp(rho))
     assert rc == 0
     return float(loss[0]), rho, gx.reshape(n, 3), gv.reshape(n, 3), float(gnu[0])
+
+
+def oracle_adjoint_f32_trajectory(params, grids, x0, v0, steps, origin, spacing, dims, target, nthreads=None):
+    """FP64 adjoint of the FP32 oracle trajectory (the GPU's, bit for bit):
+    (loss, d loss/d x0, d loss/d v0, d loss/d viscosity)."""
+    o32 = OracleSim(params, grids, nthreads=nthreads)
+    o32.set_state(x0, v0)
+    o64 = OracleSim(params, grids, fp64=True, nthreads=nthreads)
+    g = GridC()
+    g.origin[:] = [float(v) for v in origin]
+    g.spacing[:] = [float(v) for v in spacing]
+    g.dims[:] = [int(v) for v in dims]
+    n = int(params.n_particles)
+    t = np.ascontiguousarray(target, np.float64)
+    loss, gnu = np.zeros(1), np.zeros(1)
+    gx, gv = np.empty(3 * n), np.empty(3 * n)
+    rc = ORC.orc_adjoint_run_f32_trajectory(o32.h, o64.h, int(steps), C.addressof(g), dp(t), dp(loss), dp(gx), dp(gv),
+                                            dp(gnu))
+    assert rc == 0
+    return float(loss[0]), gx.reshape(n, 3), gv.reshape(n, 3), float(gnu[0])
 
 
 def oracle_keys(x, cell, table):

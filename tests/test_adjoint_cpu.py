@@ -48,3 +48,22 @@ def test_host_adjoint_matches_fp64_finite_differences():
     nu = float(p.viscosity_strenght)
     fd = (loss(x0, v0, nu + 1e-7) - loss(x0, v0, nu - 1e-7)) / 2e-7
     assert gnu == pytest.approx(fd, rel=1e-2)
+
+
+def test_fp64_adjoint_of_fp32_trajectory_close_to_fp64_adjoint():
+    """The widened-tape variant runs the same FP64 adjoint on the FP32 oracle's
+    trajectory; over a short run the two trajectories, and so the gradients,
+    stay within FP32 drift."""
+    w = CF.dam_break(side=(12, 12, 12), map_res=24)
+    grids = [H.oracle_volume_map(k, sp, r, w.h) for k, sp, r in w.shapes]
+    p = V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos)
+    dims = (24, 24, 24)
+    spacing = tuple(b / (d - 1) for b, d in zip(w.box_size, dims))
+    target = np.zeros(int(np.prod(dims)))
+    L, gx, gv, gnu = H.oracle_adjoint_f32_trajectory(p, grids, w.x, w.v, 10, (0.0, 0.0, 0.0), spacing, dims, target)
+    L64, _, hx, hv, hnu = H.oracle_adjoint(p, grids, w.x.astype(np.float64), w.v.astype(np.float64), 10,
+                                           (0.0, 0.0, 0.0), spacing, dims, target)
+    assert L == pytest.approx(L64, rel=1e-5)
+    assert np.linalg.norm(gv - hv) / np.linalg.norm(hv) < 1e-2
+    assert np.linalg.norm(gx - hx) / np.linalg.norm(hx) < 1e-2
+    assert gnu == pytest.approx(hnu, rel=1e-2)

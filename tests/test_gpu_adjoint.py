@@ -115,6 +115,26 @@ def test_adjoint_matches_fp64_host_adjoint(steps):
     assert np.sign(gnu) == np.sign(hnu) and abs(gnu - hnu) <= 0.3 * abs(hnu)
 
 
+@pytest.mark.parametrize("steps", [20, 200])
+def test_adjoint_matches_fp64_adjoint_of_the_same_trajectory(steps):
+    """The GPU adjoint against the oracle's FP64 adjoint run on the FP32
+    trajectory, which the GPU reproduces bit for bit: the adjoint arithmetic
+    alone, without the divergence of FP32 and FP64 trajectories."""
+    w, grids, p, sim, adj, target, spacing, dims = _setup(steps=steps, every=min(steps, 20))
+    loss, _ = adj.forward(w.x, w.v, target)
+    gx, gv, gnu = adj.backward()
+    L, mx, mv, mnu = H.oracle_adjoint_f32_trajectory(p, grids, w.x, w.v, steps, (0.0, 0.0, 0.0), spacing, dims,
+                                                     target.astype(np.float64))
+    ex = np.linalg.norm(gx - mx) / np.linalg.norm(mx)
+    ev = np.linalg.norm(gv - mv) / np.linalg.norm(mv)
+    print(f"{steps} substeps, same trajectory: x0 {ex:.2e}, v0 {ev:.2e}, viscosity {gnu:.6e} vs {mnu:.6e}")
+    # the GPU sums FP32 kernel values in double, the oracle evaluates the kernel
+    # in double; the loss, a small difference of densities, magnifies that
+    assert loss == pytest.approx(L, rel=1e-3)
+    assert ex < 5e-3 and ev < 5e-3
+    assert gnu == pytest.approx(mnu, rel=2e-2)
+
+
 def test_torch_autograd_function_wrapper():
     """tests/torch_ext (a torch::autograd::Function over vfs_adjoint_*) gives
     the loss and gradients of the C ABI path.  Runs in a child process that
