@@ -144,7 +144,8 @@ int vfs_sim_create(const vfs_params *params, const vfs_boundary_object *objects,
 /* Replaces SPHModel::Clear / WCSPHWithBoundaryModel buffers Destroy (model.cpp:102-111). */
 int vfs_sim_destroy(vfs_sim *sim);
 /* Replaces SPHModel::SetParticleState (model.cpp:139-147): AoS float3 host arrays,
- * densities of the range are zeroed.  count < 0 means n_particles - offset. */
+ * densities of the range are zeroed.  count < 0 means n_particles - offset.
+ * With VFS_FLAG_TRACK_IDS the particle ids restart at the current indices. */
 int vfs_sim_set_state(vfs_sim *sim, const float *positions, const float *velocities,
                       uint32_t offset, int64_t count);
 /* Runtime parameter update (the ImGui sliders: model.cpp:236-264,

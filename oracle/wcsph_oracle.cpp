@@ -1607,6 +1607,7 @@ void orc_sim_set_state(void* h, const float* x, const float* v, uint32_t offset,
             sim.v[3 * (size_t)offset + i] = v[i];
         }
         for (uint32_t i = 0; i < count; i++) sim.rho[offset + i] = 0;
+        for (uint32_t i = 0; i < sim.n; i++) sim.pid[i] = i;  // ids restart at the current indices
     };
     if (s->fp64)
         set(s->s64);
@@ -1625,6 +1626,7 @@ void orc_sim_set_state_f64(void* h, const double* x, const double* v, uint32_t o
             sim.v[3 * (size_t)offset + i] = (R)v[i];
         }
         for (uint32_t i = 0; i < count; i++) sim.rho[offset + i] = 0;
+        for (uint32_t i = 0; i < sim.n; i++) sim.pid[i] = i;
     };
     if (s->fp64)
         set(s->s64);

@@ -559,14 +559,18 @@ int autotune_impl(vfs_sim* sim, uint32_t substeps_per_trial, bool allow_cell_siz
 int profile_impl(vfs_sim* sim, uint32_t n, float* ms, uint32_t* launches, const char** names, uint32_t max_stages,
                  uint32_t* n_stages) {
     VFS_CHECK(sim && n_stages, VFS_ERR_INVALID, "null argument");
-    static const char* kNames[] = {"predict",     "sort_prepare", "sort_histogram", "sort_pass",
-                                   "reorder",     "cell_ranges",  "density",        "forces_integrate"};
+    static const char* kNames[] = {"predict", "sort_prepare", "sort_scan",   "sort_histogram",
+                                   "sort_pass", "sort_merge", "reorder",     "cell_ranges",
+                                   "density", "forces_integrate"};
     constexpr uint32_t kN = sizeof(kNames) / sizeof(kNames[0]);
     *n_stages = 0;
     if (sim->n == 0 || n == 0) return 0;
     VFS_TRY(sim->ensure_keys(sim->stream));
     EventTimer timer;
-    for (uint32_t k = 0; k < n; k++) VFS_TRY(sim->record_substep(sim->stream, &timer));
+    for (uint32_t k = 0; k < n; k++) {
+        VFS_TRY(sim->record_substep(sim->stream, &timer));
+        sim->resort_ready = true;
+    }
     sim->stepped = true;
     VFS_HIP(hipStreamSynchronize(sim->stream));
     if (timer.err != hipSuccess) return fail(VFS_ERR_HIP, "event creation failed");

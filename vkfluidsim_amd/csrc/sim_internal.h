@@ -55,6 +55,9 @@ struct vfs_sim {
 
     bool keys_valid = false;
     bool stepped = false;
+    // sorted_keys holds an ascending key array and cell_start its bucket
+    // starts (true after the first substep): later substeps re-sort
+    bool resort_ready = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     vfs_tuning tuning{8, 16, 128, 1, 1.0f};
@@ -112,6 +115,16 @@ struct vfs_sim {
         return 0;
     }
 
+    // Stable sort of this substep's keys (the preparation kernel also resets
+    // the offsets table to n): a re-sort against the previous substep's
+    // sorted keys once there are any.
+    int sort_substep(hipStream_t s, StageTimer* timer) {
+        if (resort_ready)
+            return resort_keys_iota(plan, keys, sorted_keys, perm, cell_start, sort_ws, s, timer, offsets,
+                                    (size_t)n + 1, n);
+        return sort_keys_iota(plan, keys, sorted_keys, perm, sort_ws, s, timer, offsets, (size_t)n + 1, n);
+    }
+
     // LagueModel::Step body (lague_model.cpp:106-122) for one substep.
     int record_lague_substep(hipStream_t s, StageTimer* timer) {
         const StepBuffers b = buffers();
@@ -119,7 +132,7 @@ struct vfs_sim {
         if (timer) timer->begin(s, "predict");
         VFS_TRY(launch_lague_predict(b, c, lague, s));
         if (timer) timer->end(s);
-        VFS_TRY(sort_keys_iota(plan, keys, sorted_keys, perm, sort_ws, s, timer, offsets, (size_t)n + 1, n));
+        VFS_TRY(sort_substep(s, timer));
         if (timer) timer->begin(s, "reorder");
         VFS_TRY(launch_reorder_state(b, s));
         if (timer) timer->end(s);
@@ -140,8 +153,7 @@ struct vfs_sim {
         const StepBuffers b = buffers();
         const SceneTable table{consts_dev, wave_scene};
         const SceneTable* tp = batch ? &table : nullptr;
-        // the sort's preparation kernel also resets the offsets table to n
-        VFS_TRY(sort_keys_iota(plan, keys, sorted_keys, perm, sort_ws, s, timer, offsets, (size_t)n + 1, n));
+        VFS_TRY(sort_substep(s, timer));
         if (timer) timer->begin(s, "reorder");
         VFS_TRY(launch_reorder_state(b, s));  // + key starts into the offsets table
         if (timer) timer->end(s);
@@ -248,8 +260,14 @@ struct vfs_sim {
         if (n == 0 || count == 0) return 0;
         if (s != stream) VFS_TRY(order_after(s, stream));
         VFS_TRY(ensure_keys(s));
+        if (!resort_ready) {
+            // the first substep sorts from scratch; the graph records the re-sort
+            VFS_TRY(record_substep(s, nullptr));
+            resort_ready = true;
+            count--;
+        }
         if (!(flags & VFS_FLAG_NO_GRAPH)) {
-            VFS_TRY(ensure_graph());
+            if (count) VFS_TRY(ensure_graph());
             for (uint32_t k = 0; k < count; k++) VFS_HIP(hipGraphLaunch(exec, s));
         } else {
             for (uint32_t k = 0; k < count; k++) VFS_TRY(record_substep(s, nullptr));

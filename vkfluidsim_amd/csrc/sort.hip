@@ -82,9 +82,11 @@ __global__ __launch_bounds__(256) void k_zero_words(uint32_t* __restrict__ a, si
 // Digit histograms of every pass in one read of the keys.
 template <int BITS>
 __global__ __launch_bounds__(256) void k_radix_histogram(const uint32_t* __restrict__ keys, uint32_t n,
-                                                         int passes, uint32_t* __restrict__ hist) {
+                                                         int passes, uint32_t* __restrict__ hist,
+                                                         const uint32_t* __restrict__ n_dev) {
     constexpr int RADIX = 1 << BITS;
     __shared__ uint32_t sh[kMaxPasses * RADIX];
+    if (n_dev) n = *n_dev;  // element count known on the device only
     for (int i = threadIdx.x; i < passes * RADIX; i += kThreads) sh[i] = 0;
     __syncthreads();
     for (uint32_t idx = blockIdx.x * kThreads + threadIdx.x; idx < n; idx += gridDim.x * kThreads) {
@@ -109,7 +111,8 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ vals_out, uint32_t n, int shift,
                                                     const uint32_t* __restrict__ hist,
                                                     uint32_t* __restrict__ status,
-                                                    uint32_t* __restrict__ tile_counter) {
+                                                    uint32_t* __restrict__ tile_counter,
+                                                    const uint32_t* __restrict__ n_dev) {
     constexpr int RADIX = 1 << BITS;
     constexpr int DPT = RADIX / kThreads;  // digits per thread
     constexpr int TILE = kThreads * ITEMS;
@@ -129,6 +132,10 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t base = tile * (uint32_t)TILE;
+    // With a device-side count, tiles past it have nothing to sort; tiles are
+    // claimed in order and look back only at earlier ones, so they just leave.
+    if (n_dev) n = *n_dev;
+    if (base >= n) return;
     const uint32_t wave_base = base + (uint32_t)w * 64u * ITEMS;
 
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
@@ -443,6 +450,134 @@ __global__ __launch_bounds__(256) void k_scan_lookback(uint32_t* __restrict__ da
     }
 }
 
+// ------------------------------------------------------------ re-sort
+// Between substeps few particles change cell, so the new keys mostly equal
+// the previous substep's sorted keys at the same index ("unchanged").  The
+// previous sorted keys ascend, so the unchanged particles already form a run
+// sorted by (key, index); the changed ones (set B, compacted in index order)
+// are sorted by the radix kernels with a device-side count, and both runs are
+// merged by position:
+//   unchanged j:  out = (j - #B before j) + #{b in B : (key_b, b) < (key_j, j)}
+//   B entry r:    out = (c - #B before c) + r, with c = clamp(b, L, U) and
+//                 [L, U) the previous bucket of key_b (previous cell starts)
+// which is exactly the stable sort, for any number of changed keys.
+
+// Changed keys per tile.
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_resort_count(const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ old_keys, uint32_t n,
+                                                      uint32_t* __restrict__ tile_counts) {
+    constexpr int TILE = kThreads * ITEMS;
+    __shared__ uint32_t s_tmp[kWaves];
+    const uint32_t base = blockIdx.x * (uint32_t)TILE;
+    uint32_t c = 0;
+    for (int i = threadIdx.x; i < TILE; i += kThreads) {
+        const uint32_t idx = base + (uint32_t)i;
+        c += (idx < n && keys[idx] != old_keys[idx]) ? 1u : 0u;
+    }
+    uint32_t total;
+    (void)block_exclusive_scan(c, s_tmp, total);
+    if (threadIdx.x == 0) tile_counts[blockIdx.x] = total;
+}
+
+// Exclusive count nb of changed keys before each index (tile prefix from the
+// tile counts, then a block scan), the changed keys and indices compacted in
+// index order, and on the last tile the total in state[0] and nb[n].
+template <int ITEMS>
+__global__ __launch_bounds__(256) void k_resort_compact(const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ old_keys, uint32_t n,
+                                                        const uint32_t* __restrict__ tile_counts,
+                                                        uint32_t* __restrict__ nb, uint32_t* __restrict__ bkeys,
+                                                        uint32_t* __restrict__ bvals, uint32_t* __restrict__ state) {
+    constexpr int TILE = kThreads * ITEMS;
+    __shared__ uint8_t s_flag[TILE];
+    __shared__ uint32_t s_data[TILE];
+    __shared__ uint32_t s_tmp[kWaves];
+    const int tid = threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * (uint32_t)TILE;
+    uint32_t before = 0;
+    for (uint32_t t = (uint32_t)tid; t < tile; t += kThreads) before += tile_counts[t];
+    uint32_t prefix;
+    (void)block_exclusive_scan(before, s_tmp, prefix);
+    for (int i = tid; i < TILE; i += kThreads) {
+        const uint32_t idx = base + (uint32_t)i;
+        s_flag[i] = (idx < n && keys[idx] != old_keys[idx]) ? 1 : 0;
+    }
+    __syncthreads();
+    uint32_t local[ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        local[j] = s_flag[tid * ITEMS + j];
+        sum += local[j];
+    }
+    uint32_t total;
+    uint32_t run = prefix + block_exclusive_scan(sum, s_tmp, total);
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        s_data[tid * ITEMS + j] = run;
+        run += local[j];
+    }
+    __syncthreads();
+    for (int i = tid; i < TILE; i += kThreads) {
+        const uint32_t idx = base + (uint32_t)i;
+        if (idx >= n) continue;
+        const uint32_t e = s_data[i];
+        nb[idx] = e;
+        if (s_flag[i]) {
+            bkeys[e] = keys[idx];
+            bvals[e] = idx;
+        }
+    }
+    if (tile == gridDim.x - 1u && tid == 0) {
+        nb[n] = prefix + total;
+        state[0] = prefix + total;
+    }
+}
+
+// Output positions of both runs: threads below the changed count place the
+// sorted changed entry of their index, every unchanged index finds how many
+// changed entries precede it by binary search.
+__global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict__ keys,
+                                                      const uint32_t* __restrict__ nb, uint32_t n,
+                                                      const uint32_t* __restrict__ state,
+                                                      const uint32_t* __restrict__ bks,
+                                                      const uint32_t* __restrict__ bis,
+                                                      const uint32_t* __restrict__ cell_start,
+                                                      uint32_t* __restrict__ keys_out, uint32_t* __restrict__ perm_out) {
+    const uint32_t g = blockIdx.x * (uint32_t)kThreads + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t count = state[0];
+    if (g < count) {
+        const uint32_t kb = bks[g], ib = bis[g];
+        const uint32_t lo = cell_start[kb], hi = cell_start[kb + 1u];
+        const uint32_t c = min(max(ib, lo), hi);
+        const uint32_t pos = (c - nb[c]) + g;
+        if (pos < n) {  // always, for an ascending previous key array and its bucket starts
+            keys_out[pos] = kb;
+            perm_out[pos] = ib;
+        }
+    }
+    const uint32_t e0 = nb[g];
+    if (nb[g + 1u] != e0) return;  // changed: placed by its sorted entry
+    const uint32_t kj = keys[g];
+    uint32_t lo = 0, hi = count;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t km = bks[mid];
+        if (km < kj || (km == kj && bis[mid] < g))
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    const uint32_t out = (g - e0) + lo;
+    if (out < n) {
+        keys_out[out] = kj;
+        perm_out[out] = g;
+    }
+}
+
 inline uint32_t grid_for(uint64_t n, uint32_t block = 256) {
     return (uint32_t)((n + block - 1) / block);
 }
@@ -481,6 +616,11 @@ SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int i
     p.off_tmp[1] = take(n);
     p.off_tmp[2] = take(n);
     p.off_tmp[3] = take(n);
+    p.off_rs_nb = take((size_t)n + 1);
+    p.off_rs_bkeys = take(n);
+    p.off_rs_bvals = take(n);
+    p.off_rs_bks = take(n);
+    p.off_rs_bis = take(n);
     p.off_hist = take(kMaxPasses * radix);
     p.off_counters = take(kMaxPasses);
     p.off_status = take((size_t)p.passes * p.num_tiles * radix);
@@ -488,6 +628,10 @@ SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int i
     p.cs_tiles = (uint32_t)(((uint64_t)n + 1 + cs_tile - 1) / cs_tile);
     p.off_cs_counter = take(1);
     p.off_cs_status = take(p.cs_tiles);
+    const uint32_t rs_tile = (uint32_t)kThreads * kResortItems;
+    p.rs_tiles = n == 0 ? 0 : (n + rs_tile - 1) / rs_tile;
+    p.off_rs_tile_counts = take(p.rs_tiles);
+    p.off_rs_state = take(2);
     p.zero_begin = p.off_hist;
     p.workspace_words = off;
     return p;
@@ -500,32 +644,77 @@ int valid_sort_config(int bits, int items) {
 template <int BITS, int ITEMS>
 static void launch_pass(const uint32_t* ki, const uint32_t* vi, uint32_t* ko, uint32_t* vo, uint32_t n, int shift,
                         const uint32_t* hist, uint32_t* status, uint32_t* counter, uint32_t tiles,
-                        hipStream_t s) {
+                        const uint32_t* n_dev, hipStream_t s) {
     hipLaunchKernelGGL((k_radix_pass<BITS, ITEMS>), dim3(tiles), dim3(kThreads), 0, s, ki, vi, ko, vo, n, shift,
-                       hist, status, counter);
+                       hist, status, counter, n_dev);
 }
 
 template <int BITS>
 static void launch_pass_items(int items, const uint32_t* ki, const uint32_t* vi, uint32_t* ko, uint32_t* vo,
                               uint32_t n, int shift, const uint32_t* hist, uint32_t* status, uint32_t* counter,
-                              uint32_t tiles, hipStream_t s) {
+                              uint32_t tiles, const uint32_t* n_dev, hipStream_t s) {
     if (items == 8)
-        launch_pass<BITS, 8>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, s);
+        launch_pass<BITS, 8>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, n_dev, s);
     else if (items == 12)
-        launch_pass<BITS, 12>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, s);
+        launch_pass<BITS, 12>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, n_dev, s);
     else
-        launch_pass<BITS, 16>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, s);
+        launch_pass<BITS, 16>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, n_dev, s);
 }
 
-int sort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
-                   uint32_t* workspace, hipStream_t s, StageTimer* timer, uint32_t* fill, size_t fill_words,
-                   uint32_t fill_value) {
+// Stable radix sort of (keys_in, vals_in or iota) into keys_out / vals_out;
+// n_dev: element count read on the device (at most p.n).
+static int radix_sort(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
+                      uint32_t* vals_out, const uint32_t* n_dev, uint32_t* workspace, hipStream_t s,
+                      StageTimer* timer) {
     const uint32_t n = p.n;
-    if (n == 0) return 0;
     const size_t radix = (size_t)1 << p.bits;
     uint32_t* hist = workspace + p.off_hist;
     uint32_t* counters = workspace + p.off_counters;
     uint32_t* status = workspace + p.off_status;
+    if (timer) timer->begin(s, "sort_histogram");
+    const uint32_t hblocks = std::max<uint32_t>(1u, std::min<uint32_t>(grid_for(n, 256u * 16u), 1024u));
+    if (p.bits == 8)
+        hipLaunchKernelGGL(k_radix_histogram<8>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist, n_dev);
+    else if (p.bits == 9)
+        hipLaunchKernelGGL(k_radix_histogram<9>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist, n_dev);
+    else
+        hipLaunchKernelGGL(k_radix_histogram<10>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist,
+                           n_dev);
+    VFS_TRY(check_launch("sort_histogram"));
+    if (timer) timer->end(s);
+    uint32_t* tmpk[2] = {workspace + p.off_tmp[0], workspace + p.off_tmp[2]};
+    uint32_t* tmpv[2] = {workspace + p.off_tmp[1], workspace + p.off_tmp[3]};
+    for (int pass = 0; pass < p.passes; pass++) {
+        const uint32_t* ki = pass == 0 ? keys_in : tmpk[(pass - 1) & 1];
+        const uint32_t* vi = pass == 0 ? vals_in : tmpv[(pass - 1) & 1];
+        const bool last = pass == p.passes - 1;
+        uint32_t* ko = last ? keys_out : tmpk[pass & 1];
+        uint32_t* vo = last ? vals_out : tmpv[pass & 1];
+        uint32_t* st = status + (size_t)pass * p.num_tiles * radix;
+        if (timer) timer->begin(s, "sort_pass");
+        if (p.bits == 8)
+            launch_pass_items<8>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
+                                 counters + pass, p.num_tiles, n_dev, s);
+        else if (p.bits == 9)
+            launch_pass_items<9>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
+                                 counters + pass, p.num_tiles, n_dev, s);
+        else
+            launch_pass_items<10>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
+                                  counters + pass, p.num_tiles, n_dev, s);
+        VFS_TRY(check_launch("sort_pass"));
+        if (timer) timer->end(s);
+    }
+    return 0;
+}
+
+// Preparation (zeroed sort workspace, filled offsets table), then the radix
+// sort of all keys, or (cell_start != nullptr) the re-sort against the
+// previous sorted keys in keys_out and their bucket starts.
+static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
+                     const uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer,
+                     uint32_t* fill, size_t fill_words, uint32_t fill_value) {
+    const uint32_t n = p.n;
+    if (n == 0) return 0;
     const size_t zero_words = p.workspace_words - p.zero_begin;
     if (timer) timer->begin(s, "sort_prepare");
     hipLaunchKernelGGL(k_prepare, dim3(std::min<uint32_t>(grid_for(std::max(zero_words, fill_words)), 2048u)),
@@ -533,39 +722,41 @@ int sort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_ou
                        fill_value);
     VFS_TRY(check_launch("sort_prepare"));
     if (timer) timer->end(s);
-    if (timer) timer->begin(s, "sort_histogram");
-    const uint32_t hblocks = std::max<uint32_t>(1u, std::min<uint32_t>(grid_for(n, 256u * 16u), 1024u));
-    if (p.bits == 8)
-        hipLaunchKernelGGL(k_radix_histogram<8>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist);
-    else if (p.bits == 9)
-        hipLaunchKernelGGL(k_radix_histogram<9>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist);
-    else
-        hipLaunchKernelGGL(k_radix_histogram<10>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist);
-    VFS_TRY(check_launch("sort_histogram"));
+    if (!cell_start) return radix_sort(p, keys_in, nullptr, keys_out, perm_out, nullptr, workspace, s, timer);
+
+    uint32_t* nb = workspace + p.off_rs_nb;
+    uint32_t* bkeys = workspace + p.off_rs_bkeys;
+    uint32_t* bvals = workspace + p.off_rs_bvals;
+    uint32_t* bks = workspace + p.off_rs_bks;
+    uint32_t* bis = workspace + p.off_rs_bis;
+    uint32_t* counts = workspace + p.off_rs_tile_counts;
+    uint32_t* state = workspace + p.off_rs_state;
+    if (timer) timer->begin(s, "sort_scan");
+    hipLaunchKernelGGL(k_resort_count<kResortItems>, dim3(p.rs_tiles), dim3(256), 0, s, keys_in, keys_out, n, counts);
+    VFS_TRY(check_launch("sort_scan"));
+    hipLaunchKernelGGL(k_resort_compact<kResortItems>, dim3(p.rs_tiles), dim3(256), 0, s, keys_in, keys_out, n,
+                       counts, nb, bkeys, bvals, state);
+    VFS_TRY(check_launch("sort_scan"));
     if (timer) timer->end(s);
-    uint32_t* tmpk[2] = {workspace + p.off_tmp[0], workspace + p.off_tmp[2]};
-    uint32_t* tmpv[2] = {workspace + p.off_tmp[1], workspace + p.off_tmp[3]};
-    for (int pass = 0; pass < p.passes; pass++) {
-        const uint32_t* ki = pass == 0 ? keys_in : tmpk[(pass - 1) & 1];
-        const uint32_t* vi = pass == 0 ? nullptr : tmpv[(pass - 1) & 1];
-        const bool last = pass == p.passes - 1;
-        uint32_t* ko = last ? keys_out : tmpk[pass & 1];
-        uint32_t* vo = last ? perm_out : tmpv[pass & 1];
-        uint32_t* st = status + (size_t)pass * p.num_tiles * radix;
-        if (timer) timer->begin(s, "sort_pass");
-        if (p.bits == 8)
-            launch_pass_items<8>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
-                                 counters + pass, p.num_tiles, s);
-        else if (p.bits == 9)
-            launch_pass_items<9>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
-                                 counters + pass, p.num_tiles, s);
-        else
-            launch_pass_items<10>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
-                                  counters + pass, p.num_tiles, s);
-        VFS_TRY(check_launch("sort_pass"));
-        if (timer) timer->end(s);
-    }
+    VFS_TRY(radix_sort(p, bkeys, bvals, bks, bis, state, workspace, s, timer));
+    if (timer) timer->begin(s, "sort_merge");
+    hipLaunchKernelGGL(k_resort_merge, dim3(grid_for(n)), dim3(256), 0, s, keys_in, nb, n, state, bks, bis,
+                       cell_start, keys_out, perm_out);
+    VFS_TRY(check_launch("sort_merge"));
+    if (timer) timer->end(s);
     return 0;
+}
+
+int sort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
+                   uint32_t* workspace, hipStream_t s, StageTimer* timer, uint32_t* fill, size_t fill_words,
+                   uint32_t fill_value) {
+    return sort_impl(p, keys_in, keys_out, perm_out, nullptr, workspace, s, timer, fill, fill_words, fill_value);
+}
+
+int resort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
+                     const uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer,
+                     uint32_t* fill, size_t fill_words, uint32_t fill_value) {
+    return sort_impl(p, keys_in, keys_out, perm_out, cell_start, workspace, s, timer, fill, fill_words, fill_value);
 }
 
 int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,

@@ -31,10 +31,15 @@ struct SortPlan {
     size_t off_hist, off_counters, off_status;
     uint32_t cs_tiles;                  // tiles of the cell-start scan over n + 1 entries
     size_t off_cs_counter, off_cs_status;
+    // re-sort (changed keys sorted and merged into the previous order)
+    uint32_t rs_tiles;
+    size_t off_rs_nb, off_rs_bkeys, off_rs_bvals, off_rs_bks, off_rs_bis;
+    size_t off_rs_tile_counts, off_rs_state;
     size_t zero_begin;
     size_t workspace_words;
 };
 
+constexpr int kResortItems = 16;  // keys per thread of the re-sort's count and compaction
 SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int items);
 int valid_sort_config(int bits, int items);
 
@@ -45,6 +50,13 @@ int valid_sort_config(int bits, int items);
 int sort_keys_iota(const SortPlan& plan, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
                    uint32_t* workspace, hipStream_t s, StageTimer* timer, uint32_t* fill = nullptr,
                    size_t fill_words = 0, uint32_t fill_value = 0);
+// The same stable sort when keys_out holds an ascending key array (the
+// previous substep's sorted keys) and cell_start its bucket starts: keys
+// equal to the previous key at their index keep their relative order, the
+// changed ones are radix-sorted on their own (device-side count) and merged.
+int resort_keys_iota(const SortPlan& plan, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
+                     const uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer,
+                     uint32_t* fill = nullptr, size_t fill_words = 0, uint32_t fill_value = 0);
 
 // keys[i] = key_base + HashCell3D(floor(pos_i / cell)) % table_size
 int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,
