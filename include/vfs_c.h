@@ -50,7 +50,7 @@ typedef enum vfs_status {
  */
 typedef struct vfs_params {
     float gravity[3];
-    float smooth_radius;
+    float smooth_radius;      /* in [2^-40, 2^40] (VFS_ERR_INVALID otherwise) */
     float time_scale;
     int32_t iterations;       /* substeps per vfs_sim_step */
     int32_t n_particles;

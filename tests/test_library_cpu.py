@@ -45,6 +45,12 @@ def test_sim_create_rejects_bad_parameters():
     p = V.make_params(10, 0.04, dt=1e-4, cell_size=0.02)
     rc, msg = _err(V.lib.vfs_sim_create, C.byref(p), None, 0, C.byref(out))
     assert rc == -1 and "cell_size" in msg
+    # the smoothing radius keeps the force terms' divisors where the force
+    # pass's reciprocal is exact
+    for h in (0.0, 2.0 ** -41, 2.0 ** 41, float("inf"), float("nan")):
+        p = V.make_params(10, h, dt=1e-4, cell_size=1.0)
+        rc, msg = _err(V.lib.vfs_sim_create, C.byref(p), None, 0, C.byref(out))
+        assert rc == -1 and "smooth_radius" in msg
     p = V.make_params(10, 0.04, dt=1e-4)
     rc, msg = _err(V.lib.vfs_sim_create, C.byref(p), None, 1, C.byref(out))
     assert rc == -1 and "objects" in msg

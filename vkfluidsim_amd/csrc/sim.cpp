@@ -147,8 +147,10 @@ int validate_params(const vfs_params* p) {
     VFS_CHECK(p->n_particles >= 0 && p->n_particles < (1 << 28), VFS_ERR_INVALID,
               "n_particles must be in [0, 2^28)");
     VFS_CHECK(p->iterations >= 1, VFS_ERR_INVALID, "iterations must be >= 1");
-    VFS_CHECK(p->smooth_radius > 0.0f && std::isfinite(p->smooth_radius), VFS_ERR_INVALID,
-              "smooth_radius must be positive");
+    // the force pass's reciprocals are exact for divisors in [2^-100, 2^100],
+    // which r in (0, h] and r^2 + 0.01 h^2 stay in for h in this range
+    VFS_CHECK(p->smooth_radius >= 0x1p-40f && p->smooth_radius <= 0x1p40f, VFS_ERR_INVALID,
+              "smooth_radius must lie in [2^-40, 2^40]");
     VFS_CHECK(p->cell_size >= 0.0f && std::isfinite(p->cell_size), VFS_ERR_INVALID, "cell_size must be >= 0");
     const float cell = p->cell_size > 0.0f ? p->cell_size : p->smooth_radius;
     VFS_CHECK(cell >= p->smooth_radius, VFS_ERR_INVALID,

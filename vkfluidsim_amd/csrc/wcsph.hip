@@ -30,7 +30,7 @@ namespace {
 // no branch.  A queue can therefore hold up to kQueue - 1 + kUnroll entries
 // before the flush check.
 constexpr int kUnroll = (int)kCandidateGroup;
-constexpr int kQueue = 16;
+constexpr int kQueue = 8;
 constexpr int kQueueCap = kQueue + kUnroll;
 
 // Row (dz, dy) of the 27-cell stencil: the buckets of cells (cx-1..cx+1, cy+dy,
@@ -251,17 +251,26 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, 
     if (listed <= c.nbr_cap) {
         // the lane's own list, just written (same-address accesses of a wave
         // are ordered); entries past listed in the last chunk are ignored
+        // (one chunk's positions in flight while the previous chunk is summed;
+        // chunk reads past the last one are clamped to it)
         const uint4* __restrict__ own = list + i;
         const uint32_t groups = (listed + 3u) >> 2;
+        const uint32_t last = groups ? groups - 1u : 0u;
+        uint4 en = own[(size_t)min(1u, last) * b.n];
+        float4 q0, q1, q2, q3;
+        {
+            const uint4 e = own[0];
+            q0 = load_f4(xr, e.x), q1 = load_f4(xr, e.y), q2 = load_f4(xr, e.z), q3 = load_f4(xr, e.w);
+        }
         for (uint32_t gi = 0; gi < groups; gi++) {
-            const uint4 e = own[(size_t)gi * b.n];
-            const float4 q0 = load_f4(xr, e.x), q1 = load_f4(xr, e.y), q2 = load_f4(xr, e.z),
-                         q3 = load_f4(xr, e.w);
+            const float4 p0 = q0, p1 = q1, p2 = q2, p3 = q3;
+            q0 = load_f4(xr, en.x), q1 = load_f4(xr, en.y), q2 = load_f4(xr, en.z), q3 = load_f4(xr, en.w);
+            en = own[(size_t)min(gi + 2u, last) * b.n];
             const uint32_t t = 4u * gi;
-            add(q0);
-            if (t + 1u < listed) add(q1);
-            if (t + 2u < listed) add(q2);
-            if (t + 3u < listed) add(q3);
+            add(p0);
+            if (t + 1u < listed) add(p1);
+            if (t + 2u < listed) add(p2);
+            if (t + 3u < listed) add(p3);
         }
     } else {
         cells([&](const RowRanges& r) {
@@ -303,7 +312,7 @@ __device__ __forceinline__ void force_pair(ForceAcc& f, const float4& p4, const 
     const float r = sqrtf(r2);
     float nx, ny, nz;
     if (r > 0.0f) {
-        const float inv_r = 1.0f / r;
+        const float inv_r = rcp_rn(r);  // r in (0, h]: IEEE 1 / r
         nx = xx * inv_r;
         ny = xy * inv_r;
         nz = xz * inv_r;
@@ -319,7 +328,7 @@ __device__ __forceinline__ void force_pair(ForceAcc& f, const float4& p4, const 
     f.pfz += s * nz;
     const float ux = v4.x - vj.x, uy = v4.y - vj.y, uz = v4.z - vj.z;
     const float sv = vj.w * dot3(ux, uy, uz, xx, xy, xz);
-    const float inv_den = 1.0f / (r2 + c.visc_eps_h2);
+    const float inv_den = rcp_rn(r2 + c.visc_eps_h2);  // in [0.01 h^2, 1.01 h^2]: IEEE 1 / (.)
     f.vfx += (sv * (nx * G)) * inv_den;
     f.vfy += (sv * (ny * G)) * inv_den;
     f.vfz += (sv * (nz * G)) * inv_den;
@@ -348,22 +357,30 @@ __global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimCo
     if (count <= c.nbr_cap) {
         // chunks of four neighbours; entries past count in the last chunk are
         // earlier neighbours or 0, loaded and ignored; self entries are skipped
-        // as in the reference loops
+        // as in the reference loops.  The next chunk's neighbours are loaded
+        // while the current chunk's pairs are evaluated (reads past the last
+        // chunk are clamped to it).
         const uint4* __restrict__ list = reinterpret_cast<const uint4*>(b.nbr) + i;
         const uint32_t groups = (count + 3u) >> 2;
         uint4 e = groups ? list[0] : make_uint4(0u, 0u, 0u, 0u);
+        float4 q0 = load_f4(xr, e.x), w0 = load_f4(vr, e.x);
+        float4 q1 = load_f4(xr, e.y), w1 = load_f4(vr, e.y);
+        float4 q2 = load_f4(xr, e.z), w2 = load_f4(vr, e.z);
+        float4 q3 = load_f4(xr, e.w), w3 = load_f4(vr, e.w);
         for (uint32_t g = 0; g < groups; g++) {
             const uint4 en = list[(size_t)min(g + 1u, groups - 1u) * b.n];
-            const float4 q0 = load_f4(xr, e.x), w0 = load_f4(vr, e.x);
-            const float4 q1 = load_f4(xr, e.y), w1 = load_f4(vr, e.y);
-            const float4 q2 = load_f4(xr, e.z), w2 = load_f4(vr, e.z);
-            const float4 q3 = load_f4(xr, e.w), w3 = load_f4(vr, e.w);
+            const float4 n0 = load_f4(xr, en.x), m0 = load_f4(vr, en.x);
+            const float4 n1 = load_f4(xr, en.y), m1 = load_f4(vr, en.y);
+            const float4 n2 = load_f4(xr, en.z), m2 = load_f4(vr, en.z);
+            const float4 n3 = load_f4(xr, en.w), m3 = load_f4(vr, en.w);
             const uint32_t t = 4u * g;
             if (e.x != i) force_pair(f, p4, v4, pdi, q0, w0, c);
             if (t + 1u < count && e.y != i) force_pair(f, p4, v4, pdi, q1, w1, c);
             if (t + 2u < count && e.z != i) force_pair(f, p4, v4, pdi, q2, w2, c);
             if (t + 3u < count && e.w != i) force_pair(f, p4, v4, pdi, q3, w3, c);
             e = en;
+            q0 = n0, q1 = n1, q2 = n2, q3 = n3;
+            w0 = m0, w1 = m1, w2 = m2, w3 = m3;
         }
     } else {
         // overflow: the 27 cells in reference order, pairs evaluated as found

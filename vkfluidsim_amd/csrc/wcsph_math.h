@@ -15,6 +15,20 @@ __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, fl
     return __builtin_fmaf(az, bz, __builtin_fmaf(ay, by, ax * bx));
 }
 
+// 1 / x rounded to nearest, as the IEEE division 1.0f / x, for x in
+// [kRcpMin, kRcpMax]: the hardware reciprocal (within one ulp) refined by one
+// Newton-Raphson step with fused residual, y + y (1 - x y) (Markstein's
+// reciprocal iteration).  Three instructions instead of the division's
+// eleven; tests/test_gpu_arith.py compares it with 1.0f / x for every float in
+// that range.  Callers keep their operands inside it (sim.cpp bounds the
+// smoothing radius so that the force terms' divisors do).
+constexpr float kRcpMin = 0x1p-100f, kRcpMax = 0x1p100f;
+__device__ __forceinline__ float rcp_rn(float x) {
+    const float y = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, y, 1.0f);
+    return __builtin_fmaf(e, y, y);
+}
+
 // CubicSpline, kernels_3d.slang:62-76
 __device__ __forceinline__ float kernel_w(float r, const SimConsts& c) {
     const float q = r * c.inv_h;
