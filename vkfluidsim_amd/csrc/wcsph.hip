@@ -22,9 +22,11 @@ namespace vfs {
 namespace {
 
 // The cell walk appends in-range neighbours to a per-lane LDS queue and
-// moves them into the neighbour list once any active lane holds kQueue of
-// them, so the list stores run with most lanes busy.  Queued neighbours keep
-// their arrival order (the reference order).  Candidates are read kUnroll at
+// moves its complete chunks of four into the neighbour list (one 16-byte
+// store each) once any active lane holds kQueue of them, so the list stores
+// run with most lanes busy; the unstored remainder (up to three) moves to the
+// front of the queue.  Queued neighbours keep their arrival order (the
+// reference order).  Candidates are read kUnroll at
 // a time (all loads issued before the tests); every candidate writes the next
 // queue slot and only advances the count when it is in range, so the test has
 // no branch.  A queue can therefore hold up to kQueue - 1 + kUnroll entries
@@ -32,6 +34,7 @@ namespace {
 constexpr int kUnroll = (int)kCandidateGroup;
 constexpr int kQueue = 8;
 constexpr int kQueueCap = kQueue + kUnroll;
+static_assert(kQueueCap % 4 == 0, "the flush reads the three slots after the last complete chunk");
 
 // Row (dz, dy) of the 27-cell stencil: the buckets of cells (cx-1..cx+1, cy+dy,
 // cz+dz) as up to three [begin, end) ranges of sorted indices, merged into one
@@ -185,28 +188,31 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, 
     const float4* __restrict__ xs = b.xs;
     const auto xr = make_rsrc(xs, b.n * 16u);
     uint4* __restrict__ list = reinterpret_cast<uint4*>(b.nbr);
-    uint32_t e0 = 0, e1 = 0, e2 = 0, e3 = 0;  // the list chunk being filled
-    int cnt = 0;
-    uint32_t listed = 0;
+    int cnt = 0;          // queued neighbours, the unstored part of the last chunk first
+    uint32_t chunks = 0;  // complete chunks of four moved out of the queue
     auto hit = [&](uint32_t j, const float4& q, bool valid) {
         const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
         const float r2 = dot3(ox, oy, oz, ox, oy, oz);
         s_j[cnt][tid] = j;
         cnt += (valid && !(r2 > c.h2)) ? 1 : 0;
     };
-    // queued neighbours in order into the list chunks
+    // complete chunks into the list in order; the remainder's three slots move
+    // to the front whole (slots past the remainder are overwritten before use)
     auto flush = [&]() {
-        for (int t = 0; t < cnt; t++) {
-            const uint32_t j = s_j[t][tid];
-            const uint32_t k = listed & 3u;
-            e0 = k == 0u ? j : e0;
-            e1 = k == 1u ? j : e1;
-            e2 = k == 2u ? j : e2;
-            e3 = k == 3u ? j : e3;
-            if (k == 3u && listed < c.nbr_cap) list[(size_t)(listed >> 2) * b.n + i] = make_uint4(e0, e1, e2, e3);
-            listed++;
+        const int full = cnt >> 2;
+        for (int g = 0; g < full; g++) {
+            const uint4 v = make_uint4(s_j[4 * g][tid], s_j[4 * g + 1][tid], s_j[4 * g + 2][tid], s_j[4 * g + 3][tid]);
+            if (chunks < (c.nbr_cap >> 2)) list[(size_t)chunks * b.n + i] = v;
+            chunks++;
         }
-        cnt = 0;
+        const int rem = cnt & 3;
+        if (full > 0) {
+            const uint32_t r0 = s_j[4 * full][tid], r1 = s_j[4 * full + 1][tid], r2 = s_j[4 * full + 2][tid];
+            s_j[0][tid] = r0;
+            s_j[1][tid] = r1;
+            s_j[2][tid] = r2;
+        }
+        cnt = rem;
     };
     // inactive lanes (past the scene, or past the last scene of a batch) read
     // no cell starts: their rows are empty
@@ -240,7 +246,14 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, 
     });
     if (!active) return;
     flush();
-    if ((listed & 3u) != 0u && listed <= c.nbr_cap) list[(size_t)(listed >> 2) * b.n + i] = make_uint4(e0, e1, e2, e3);
+    // the last, partial chunk; its unused entries hold the particle's own index
+    const uint32_t listed = 4u * chunks + (uint32_t)cnt;
+    if (cnt > 0 && listed <= c.nbr_cap) {
+        const uint32_t a0 = s_j[0][tid];
+        const uint32_t a1 = cnt > 1 ? s_j[1][tid] : i;
+        const uint32_t a2 = cnt > 2 ? s_j[2][tid] : i;
+        list[(size_t)chunks * b.n + i] = make_uint4(a0, a1, a2, i);
+    }
     b.nbr_count[i] = listed;
 
     float density = 0.0f;
@@ -355,8 +368,8 @@ __global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimCo
 
     const uint32_t count = b.nbr_count[i];
     if (count <= c.nbr_cap) {
-        // chunks of four neighbours; entries past count in the last chunk are
-        // earlier neighbours or 0, loaded and ignored; self entries are skipped
+        // chunks of four neighbours; entries past count in the last chunk hold
+        // the particle's own index, loaded and ignored; self entries are skipped
         // as in the reference loops.  The next chunk's neighbours are loaded
         // while the current chunk's pairs are evaluated (reads past the last
         // chunk are clamped to it).
