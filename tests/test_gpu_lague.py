@@ -123,3 +123,28 @@ def test_lague_profile_stages():
                      "cell_ranges", "density", "forces_integrate"]
     with pytest.raises(V.VfsError, match="no boundary objects"):
         sim.get_boundary()
+
+
+@pytest.mark.parametrize("part", ["all", "half"])
+def test_lague_neighbour_list_overflow_bit_exact(part):
+    """Both passes walk the density pass's neighbour list; a particle with more
+    neighbours than the list holds walks the cells again, with the same
+    results.  A compressed block overflows every list ("all") or the lists of
+    its compressed half only ("half")."""
+    w = _small()
+    c = w.x.mean(axis=0)
+    sel = np.ones(w.n, bool) if part == "all" else (w.x[:, 0] < c[0])
+    w.x = w.x.copy()
+    w.x[sel] = c + (w.x[sel] - c) * np.float32(0.45)
+    print(f"case lague overflow {part}: {w.n} particles", flush=True)
+    sim, orc = _pair(w)
+    sim.substeps(1)
+    orc.substeps(1)
+    _same(sim, orc)
+    mean, mx, over = sim.neighbour_stats()
+    assert mx > 64 and over > 0
+    if part == "half":
+        assert over < w.n
+    sim.substeps(4)
+    orc.substeps(4)
+    _same(sim, orc)

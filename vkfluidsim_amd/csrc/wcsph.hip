@@ -158,43 +158,51 @@ __global__ __launch_bounds__(256) void k_reorder_state(StepBuffers b, uint32_t n
     if (i == 0 || b.sorted_keys[i - 1] != k) b.offsets[k] = i;
 }
 
-// Boundary volume, CalculateDensities (slang:153-197), the per-particle EOS
-// terms the force pass reads for every neighbour, and this substep's neighbour
-// list.  The cell walk only finds the in-range neighbours j (self included,
-// once per visit of its bucket) in reference order and stores them in chunks
-// of four, chunk-major (chunk g of particle i at g * n + i), so that list
-// reads and writes are coalesced and each store fills 16 bytes.  The density
-// sum then walks the particle's own list, every lane busy until its own count.
-// A particle with more than nbr_cap neighbours stores its count only; its
-// density, and its forces in the force pass, walk the cells again.
-template <int BLOCK, bool STAGED, bool BATCH>
-__global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, SceneTable table,
-                                                   const DevObject* __restrict__ objs, uint32_t nblocks) {
-    __shared__ uint32_t s_j[kQueueCap][BLOCK];
-    // wave slices of staged rows, padded for the masked tail group's reads
-    __shared__ float4 s_stage[STAGED ? BLOCK / 64 : 1][kWaveStage + kUnroll];
-    const int tid = threadIdx.x;
-    const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + tid;
-    const SimConsts c = scene_consts<BATCH>(c0, table, i);
-    const bool active = i - c.base < c.n;
-    const float4 p4 = b.xs[active ? i : 0u];
-    const float pos[3] = {p4.x, p4.y, p4.z};
-    float psi = 0.0f, g[3] = {0.0f, 0.0f, 0.0f};
-    if (active) boundary_volume(objs + c.obj_begin, c.obj_count, pos, c, psi, g);
-
+// The 27-cell stencil of the cell of pos as nine rows in the reference
+// order (z outer, y inner); inactive lanes read no cell starts and get empty
+// rows.
+template <class Visit>
+__device__ __forceinline__ void stencil_rows(const float* pos, bool active, const SimConsts& c,
+                                             const uint32_t* __restrict__ cs, Visit&& visit) {
     const int32_t cx = floor_to_cell(pos[0] / c.cell);
     const int32_t cy = floor_to_cell(pos[1] / c.cell);
     const int32_t cz = floor_to_cell(pos[2] / c.cell);
-    const float4* __restrict__ xs = b.xs;
-    const auto xr = make_rsrc(xs, b.n * 16u);
-    uint4* __restrict__ list = reinterpret_cast<uint4*>(b.nbr);
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) {
+            const RowRanges r = active ? row_ranges(cx, cy, cz, dy, dz, c, cs) : RowRanges{0u, 0u, 0u, 0u, 0u, 0u, 1};
+            visit(r);
+        }
+}
+
+// Shared memory of the list builder: the per-lane neighbour queues and the
+// wave slices of staged rows (padded for the masked tail group's reads).
+template <int BLOCK, bool STAGED>
+struct ListShared {
+    uint32_t queue[kQueueCap][BLOCK];
+    float4 stage[STAGED ? BLOCK / 64 : 1][kWaveStage + kUnroll];
+};
+
+// This substep's neighbour list of particle i (position pos) over the sorted
+// positions pts: the cell walk finds the neighbours j with r^2 <= h2 (self
+// included, once per visit of its bucket) in reference order and stores them
+// in chunks of four, chunk-major (chunk g of particle i at g * n + i), so that
+// list reads and writes are coalesced and each store fills 16 bytes.  Returns
+// the count; a list longer than nbr_cap stores its count only.  Every lane of
+// the wave calls it (wave votes inside); inactive lanes list nothing.
+template <int BLOCK, bool STAGED>
+__device__ __forceinline__ uint32_t build_list(ListShared<BLOCK, STAGED>& sh, const float4* __restrict__ pts,
+                                               const float* pos, float h2, bool active, uint32_t i,
+                                               const SimConsts& c, const uint32_t* __restrict__ cs,
+                                               uint4* __restrict__ list, uint32_t n) {
+    const int tid = threadIdx.x;
+    auto& s_j = sh.queue;
     int cnt = 0;          // queued neighbours, the unstored part of the last chunk first
     uint32_t chunks = 0;  // complete chunks of four moved out of the queue
     auto hit = [&](uint32_t j, const float4& q, bool valid) {
         const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
         const float r2 = dot3(ox, oy, oz, ox, oy, oz);
         s_j[cnt][tid] = j;
-        cnt += (valid && !(r2 > c.h2)) ? 1 : 0;
+        cnt += (valid && !(r2 > h2)) ? 1 : 0;
     };
     // complete chunks into the list in order; the remainder's three slots move
     // to the front whole (slots past the remainder are overwritten before use)
@@ -202,7 +210,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, 
         const int full = cnt >> 2;
         for (int g = 0; g < full; g++) {
             const uint4 v = make_uint4(s_j[4 * g][tid], s_j[4 * g + 1][tid], s_j[4 * g + 2][tid], s_j[4 * g + 3][tid]);
-            if (chunks < (c.nbr_cap >> 2)) list[(size_t)chunks * b.n + i] = v;
+            if (chunks < (c.nbr_cap >> 2)) list[(size_t)chunks * n + i] = v;
             chunks++;
         }
         const int rem = cnt & 3;
@@ -214,28 +222,18 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, 
         }
         cnt = rem;
     };
-    // inactive lanes (past the scene, or past the last scene of a batch) read
-    // no cell starts: their rows are empty
-    auto cells = [&](auto&& visit) {
-        for (int dz = -1; dz <= 1; dz++)
-            for (int dy = -1; dy <= 1; dy++) {
-                const RowRanges r = active ? row_ranges(cx, cy, cz, dy, dz, c, b.cell_start)
-                                           : RowRanges{0u, 0u, 0u, 0u, 0u, 0u, 1};
-                visit(r);
-            }
-    };
-    cells([&](const RowRanges& r) {
+    stencil_rows(pos, active, c, cs, [&](const RowRanges& r) {
         uint32_t L = 0;
         bool staged = false;
-        if (STAGED) staged = stage_row(s_stage[STAGED ? tid >> 6 : 0], r, active, xs, L);
+        if (STAGED) staged = stage_row(sh.stage[STAGED ? tid >> 6 : 0], r, active, pts, L);
         if (active) {
             auto visit = [&](uint32_t jb, uint32_t je) {
                 if (STAGED && staged)
                     walk_range(
-                        jb, je, [&](uint32_t j0, int u) { return s_stage[STAGED ? tid >> 6 : 0][j0 - L + u]; },
+                        jb, je, [&](uint32_t j0, int u) { return sh.stage[STAGED ? tid >> 6 : 0][j0 - L + u]; },
                         cnt, hit, flush);
                 else
-                    walk_range(jb, je, [&](uint32_t j0, int u) { return (xs + j0)[u]; }, cnt, hit, flush);
+                    walk_range(jb, je, [&](uint32_t j0, int u) { return (pts + j0)[u]; }, cnt, hit, flush);
             };
             visit(r.b0, r.e0);
             if (r.nr == 3) {
@@ -244,7 +242,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, 
             }
         }
     });
-    if (!active) return;
+    if (!active) return 0;
     flush();
     // the last, partial chunk; its unused entries hold the particle's own index
     const uint32_t listed = 4u * chunks + (uint32_t)cnt;
@@ -252,8 +250,60 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, 
         const uint32_t a0 = s_j[0][tid];
         const uint32_t a1 = cnt > 1 ? s_j[1][tid] : i;
         const uint32_t a2 = cnt > 2 ? s_j[2][tid] : i;
-        list[(size_t)chunks * b.n + i] = make_uint4(a0, a1, a2, i);
+        list[(size_t)chunks * n + i] = make_uint4(a0, a1, a2, i);
     }
+    return listed;
+}
+
+// Calls add(q) for the positions q (from r) of the first `listed` entries of
+// the list `own` (particle i's column) in order, every lane busy until its own
+// count.  One chunk's positions are in flight while the previous chunk is
+// summed; chunk reads past the last one are clamped to it, entries past
+// `listed` in the last chunk are loaded and ignored.  The lane's own list may
+// just have been written (same-address accesses of a wave are ordered).
+template <class Add>
+__device__ __forceinline__ void sum_list(const uint4* __restrict__ own, uint32_t listed, uint32_t n,
+                                         __amdgpu_buffer_rsrc_t r, Add&& add) {
+    const uint32_t groups = (listed + 3u) >> 2;
+    const uint32_t last = groups ? groups - 1u : 0u;
+    uint4 en = own[(size_t)min(1u, last) * n];
+    float4 q0, q1, q2, q3;
+    {
+        const uint4 e = own[0];
+        q0 = load_f4(r, e.x), q1 = load_f4(r, e.y), q2 = load_f4(r, e.z), q3 = load_f4(r, e.w);
+    }
+    for (uint32_t gi = 0; gi < groups; gi++) {
+        const float4 p0 = q0, p1 = q1, p2 = q2, p3 = q3;
+        q0 = load_f4(r, en.x), q1 = load_f4(r, en.y), q2 = load_f4(r, en.z), q3 = load_f4(r, en.w);
+        en = own[(size_t)min(gi + 2u, last) * n];
+        const uint32_t t = 4u * gi;
+        add(p0);
+        if (t + 1u < listed) add(p1);
+        if (t + 2u < listed) add(p2);
+        if (t + 3u < listed) add(p3);
+    }
+}
+
+// Boundary volume, CalculateDensities (slang:153-197), the per-particle EOS
+// terms the force pass reads for every neighbour, and this substep's neighbour
+// list (build_list).  The density sum then walks the particle's own list.  A
+// particle with more than nbr_cap neighbours stores its count only; its
+// density, and its forces in the force pass, walk the cells again.
+template <int BLOCK, bool STAGED, bool BATCH>
+__global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, SceneTable table,
+                                                   const DevObject* __restrict__ objs, uint32_t nblocks) {
+    __shared__ ListShared<BLOCK, STAGED> sh;
+    const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + threadIdx.x;
+    const SimConsts c = scene_consts<BATCH>(c0, table, i);
+    const bool active = i - c.base < c.n;
+    const float4 p4 = b.xs[active ? i : 0u];
+    const float pos[3] = {p4.x, p4.y, p4.z};
+    float psi = 0.0f, g[3] = {0.0f, 0.0f, 0.0f};
+    if (active) boundary_volume(objs + c.obj_begin, c.obj_count, pos, c, psi, g);
+
+    uint4* __restrict__ list = reinterpret_cast<uint4*>(b.nbr);
+    const uint32_t listed = build_list<BLOCK, STAGED>(sh, b.xs, pos, c.h2, active, i, c, b.cell_start, list, b.n);
+    if (!active) return;
     b.nbr_count[i] = listed;
 
     float density = 0.0f;
@@ -262,31 +312,10 @@ __global__ __launch_bounds__(BLOCK) void k_density(StepBuffers b, SimConsts c0, 
         density += c.mass * kernel_w(sqrtf(dot3(ox, oy, oz, ox, oy, oz)), c);
     };
     if (listed <= c.nbr_cap) {
-        // the lane's own list, just written (same-address accesses of a wave
-        // are ordered); entries past listed in the last chunk are ignored
-        // (one chunk's positions in flight while the previous chunk is summed;
-        // chunk reads past the last one are clamped to it)
-        const uint4* __restrict__ own = list + i;
-        const uint32_t groups = (listed + 3u) >> 2;
-        const uint32_t last = groups ? groups - 1u : 0u;
-        uint4 en = own[(size_t)min(1u, last) * b.n];
-        float4 q0, q1, q2, q3;
-        {
-            const uint4 e = own[0];
-            q0 = load_f4(xr, e.x), q1 = load_f4(xr, e.y), q2 = load_f4(xr, e.z), q3 = load_f4(xr, e.w);
-        }
-        for (uint32_t gi = 0; gi < groups; gi++) {
-            const float4 p0 = q0, p1 = q1, p2 = q2, p3 = q3;
-            q0 = load_f4(xr, en.x), q1 = load_f4(xr, en.y), q2 = load_f4(xr, en.z), q3 = load_f4(xr, en.w);
-            en = own[(size_t)min(gi + 2u, last) * b.n];
-            const uint32_t t = 4u * gi;
-            add(p0);
-            if (t + 1u < listed) add(p1);
-            if (t + 2u < listed) add(p2);
-            if (t + 3u < listed) add(p3);
-        }
+        sum_list(list + i, listed, b.n, make_rsrc(b.xs, b.n * 16u), add);
     } else {
-        cells([&](const RowRanges& r) {
+        const float4* __restrict__ xs = b.xs;
+        stencil_rows(pos, true, c, b.cell_start, [&](const RowRanges& r) {
             auto visit = [&](uint32_t jb, uint32_t je) {
                 for (uint32_t j = jb; j < je; j++) {
                     const float4 q = xs[j];
@@ -515,25 +544,37 @@ __device__ __forceinline__ void lague_walk(const float4& p, bool active, const S
 }
 
 // CalculateDensities (lague_model.slang:60-97, 222-230): density and near
-// density of the predicted positions, self included.  The force pass reads a
-// neighbour's pressure with its prediction (ps.w) and its near pressure and
-// density reciprocals from b.bnd (unused by this model): the same values the
-// reference derives per pair.
+// density of the predicted positions, self included, summed over this
+// substep's neighbour list of the predictions (build_list), which the force
+// pass walks again.  The force pass reads a neighbour's pressure with its
+// prediction (ps.w) and its near pressure and density reciprocals from b.bnd
+// (unused by this model): the same values the reference derives per pair.
+template <bool STAGED>
 __global__ __launch_bounds__(128) void k_lague_density(StepBuffers b, SimConsts c, LagueConsts l, uint32_t nblocks) {
+    __shared__ ListShared<128, STAGED> sh;
     const uint32_t i = xcd_block(nblocks) * 128u + threadIdx.x;
     const bool active = i < b.n;
     const float4 p = b.ps[active ? i : 0u];
-    float density = 0.0f, near = 0.0f;
-    lague_walk(p, active, b, c, [&](uint32_t, const float4& q, bool valid) {
-        const float ox = q.x - p.x, oy = q.y - p.y, oz = q.z - p.z;
-        const float r2 = dot3(ox, oy, oz, ox, oy, oz);
-        if (valid && !(r2 > l.h2)) {
-            const float d = sqrtf(r2);
-            density += spiky2(d, l);
-            near += spiky3(d, l);
-        }
-    });
+    const float pos[3] = {p.x, p.y, p.z};
+    uint4* __restrict__ list = reinterpret_cast<uint4*>(b.nbr);
+    const uint32_t listed = build_list<128, STAGED>(sh, b.ps, pos, l.h2, active, i, c, b.cell_start, list, b.n);
     if (!active) return;
+    b.nbr_count[i] = listed;
+    float density = 0.0f, near = 0.0f;
+    auto add = [&](const float4& q) {
+        const float ox = q.x - p.x, oy = q.y - p.y, oz = q.z - p.z;
+        const float d = sqrtf(dot3(ox, oy, oz, ox, oy, oz));
+        density += spiky2(d, l);
+        near += spiky3(d, l);
+    };
+    if (listed <= c.nbr_cap) {
+        sum_list(list + i, listed, b.n, make_rsrc(b.ps, b.n * 16u), add);
+    } else {
+        lague_walk(p, true, b, c, [&](uint32_t, const float4& q, bool valid) {
+            const float ox = q.x - p.x, oy = q.y - p.y, oz = q.z - p.z;
+            if (valid && !(dot3(ox, oy, oz, ox, oy, oz) > l.h2)) add(q);
+        });
+    }
     b.rho[i] = density;
     b.near_rho[i] = near;
     reinterpret_cast<float*>(b.ps)[4 * (size_t)i + 3] = (density - l.rho0) * l.pm;
@@ -541,23 +582,23 @@ __global__ __launch_bounds__(128) void k_lague_density(StepBuffers b, SimConsts 
 }
 
 // CalculatePressureForces (lague_model.slang:99-160, 231-238) with the
-// velocities from before this pass, then UpdatePositions with
-// ResolveCollisions (slang:162-200, 240-250).
+// velocities from before this pass, over the density pass's neighbour list
+// (self skipped; a list that overflowed walks the cells again), then
+// UpdatePositions with ResolveCollisions (slang:162-200, 240-250).
 __global__ __launch_bounds__(128) void k_lague_forces(StepBuffers b, SimConsts c, LagueConsts l, uint32_t nblocks) {
     const uint32_t i = xcd_block(nblocks) * 128u + threadIdx.x;
-    const bool active = i < b.n;
-    const uint32_t ii = active ? i : 0u;
-    const float4 p = b.ps[ii];
-    const float4 vel = b.vs[ii];
-    const float4 own = b.bnd[ii];
+    if (i >= b.n) return;
+    const float4 p = b.ps[i];
+    const float4 vel = b.vs[i];
+    const float4 own = b.bnd[i];
     const float pressure = p.w;
     const float near_pressure = own.x;
     float pfx = 0.0f, pfy = 0.0f, pfz = 0.0f, vfx = 0.0f, vfy = 0.0f, vfz = 0.0f;
-    lague_walk(p, active, b, c, [&](uint32_t j, const float4& q, bool valid) {
+    // one in-range neighbour j != i: prediction q, near pressure and density
+    // reciprocals nb, velocity vj
+    auto pair = [&](const float4& q, const float4& nb, const float4& vj) {
         const float ox = q.x - p.x, oy = q.y - p.y, oz = q.z - p.z;
         const float r2 = dot3(ox, oy, oz, ox, oy, oz);
-        if (!valid || j == i || r2 > l.h2) return;
-        const float4 nb = b.bnd[j];  // near pressure, 1 / density, 1 / near density
         const float shared = (pressure + q.w) * 0.5f;
         const float shared_near = (near_pressure + nb.x) * 0.5f;
         const float d = sqrtf(r2);
@@ -576,13 +617,39 @@ __global__ __launch_bounds__(128) void k_lague_forces(StepBuffers b, SimConsts c
         pfx += ((dx * ndd) * shared_near) * inv_nnd;
         pfy += ((dy * ndd) * shared_near) * inv_nnd;
         pfz += ((dz * ndd) * shared_near) * inv_nnd;
-        const float4 vj = b.vs[j];
         const float w = poly6(d, l);
         vfx += (vj.x - vel.x) * w;
         vfy += (vj.y - vel.y) * w;
         vfz += (vj.z - vel.z) * w;
-    });
-    if (!active) return;
+    };
+    const uint32_t count = b.nbr_count[i];
+    if (count <= c.nbr_cap) {
+        const auto pr = make_rsrc(b.ps, b.n * 16u), br = make_rsrc(b.bnd, b.n * 16u), vr = make_rsrc(b.vs, b.n * 16u);
+        const uint4* __restrict__ list = reinterpret_cast<const uint4*>(b.nbr) + i;
+        const uint32_t groups = (count + 3u) >> 2;
+        for (uint32_t g = 0; g < groups; g++) {
+            const uint4 e = list[(size_t)g * b.n];
+            const uint32_t js[4] = {e.x, e.y, e.z, e.w};
+            float4 q[4], nb[4], vj[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                q[k] = load_f4(pr, js[k]);
+                nb[k] = load_f4(br, js[k]);
+                vj[k] = load_f4(vr, js[k]);
+            }
+            const uint32_t t = 4u * g;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (t + (uint32_t)k < count && js[k] != i) pair(q[k], nb[k], vj[k]);
+        }
+    } else {
+        lague_walk(p, true, b, c, [&](uint32_t j, const float4& q, bool valid) {
+            const float ox = q.x - p.x, oy = q.y - p.y, oz = q.z - p.z;
+            const float r2 = dot3(ox, oy, oz, ox, oy, oz);
+            if (!valid || j == i || r2 > l.h2) return;
+            pair(q, b.bnd[j], b.vs[j]);
+        });
+    }
     const float inv_density = own.y;
     const float ax = pfx * inv_density + vfx * l.visc;
     const float ay = pfy * inv_density + vfy * l.visc;
@@ -713,7 +780,7 @@ int launch_lague_predict(const StepBuffers& b, const SimConsts& c, const LagueCo
 int launch_lague_density(const StepBuffers& b, const SimConsts& c, const LagueConsts& lc, hipStream_t s) {
     if (b.n == 0) return 0;
     const uint32_t nb = particle_blocks(b.n, 128);
-    hipLaunchKernelGGL(k_lague_density, dim3(nb), dim3(128), 0, s, b, c, lc, nb);
+    hipLaunchKernelGGL(k_lague_density<true>, dim3(nb), dim3(128), 0, s, b, c, lc, nb);
     return check_launch("lague_density");
 }
 
