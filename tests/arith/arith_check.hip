@@ -1,8 +1,8 @@
-// Test helper: compares the force pass's reciprocal (rcp_rn, wcsph_math.h)
-// with the IEEE division 1.0f / x for every float bit pattern in a range, on
-// the GPU.  Mode 1 compares the bare hardware reciprocal instead, which
-// differs from the division in many places: it shows that the comparison can
-// fail.
+// Test helpers of the device arithmetic in wcsph_math.h, run on the GPU.
+// rcp_check compares the force pass's reciprocal (rcp_rn) with the IEEE
+// division 1.0f / x for every float bit pattern in a range; mode 1 compares
+// the bare hardware reciprocal instead, which differs from the division in
+// many places (the comparison can fail).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -50,3 +50,4 @@ extern "C" int rcp_check(uint32_t lo_bits, uint32_t hi_bits, int mode, unsigned 
     if (d_first) (void)hipFree(d_first);
     return (int)e;
 }
+

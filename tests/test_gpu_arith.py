@@ -1,7 +1,7 @@
-"""Device arithmetic the force pass relies on: its reciprocal (hardware
-reciprocal + one Newton-Raphson step with a fused residual) equals the IEEE
-division 1.0f / x for every float in [2^-100, 2^100], the range that the
-smoothing-radius bounds keep the force terms' divisors in."""
+"""Device arithmetic the substep kernels rely on: the force pass's reciprocal
+(hardware reciprocal + one Newton-Raphson step with a fused residual) equals
+the IEEE division 1.0f / x for every float in [2^-100, 2^100], the range that
+the smoothing-radius bounds keep the force terms' divisors in."""
 import ctypes as C
 import struct
 from pathlib import Path
@@ -10,7 +10,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-LIB = Path(__file__).resolve().parent / "arith" / "librcp_check.so"
+LIB = Path(__file__).resolve().parent / "arith" / "libarith_check.so"
 
 
 def _bits(x: float) -> int:
@@ -39,3 +39,4 @@ def test_bare_hardware_reciprocal_is_caught():
     # the same comparison over [1, 2] flags the unrefined reciprocal
     bad, first = _check(1.0, 2.0, 1)
     assert bad > 1000 and 1.0 <= first <= 2.0
+
