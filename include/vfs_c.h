@@ -88,7 +88,9 @@ typedef struct vfs_boundary_object {
 /* Device addresses of a simulation's particle buffers (SPHModel::GetDataBuffers,
  * src/models/model.h:38-43).  Positions and velocities are float4 per particle
  * (x, y, z, unused); densities are float; particle_ids is NULL without
- * VFS_FLAG_TRACK_IDS; accelerations is NULL without VFS_FLAG_STORE_ACCEL. */
+ * VFS_FLAG_TRACK_IDS; accelerations is NULL without VFS_FLAG_STORE_ACCEL.
+ * sorted_keys and cell_start are read-only: the next substep re-sorts
+ * against them. */
 typedef struct vfs_device_buffers {
     float *positions;
     float *velocities;
