@@ -401,6 +401,59 @@ int vfs_trajectory_record(vfs_trajectory *trajectory, uint64_t substep, double t
 /* Writes the frame count into the header and closes the file. */
 int vfs_trajectory_close(vfs_trajectory *trajectory);
 
+/* ---- render-buffer handoff ---------------------------------------------
+ * Render buffers in the reference's data-buffer layout
+ * (SPHModel::CreateDataBuffers, src/models/model.cpp:149-156): positions,
+ * velocities and accelerations packed float3 (12 bytes per particle),
+ * densities float; particles in the simulation's current order.  The
+ * renderer reads them after vfs_sim_copy_data_buffers, in this process (its
+ * stream waits on `ready`, a HIP event recorded after each copy) or, through
+ * vfs_render_buffers_export / _open, in another one (zero copy: the other
+ * process maps the same device memory; the producer calls
+ * vfs_render_buffers_synchronize after the copy and then signals the renderer
+ * over its own channel). */
+typedef struct vfs_render_buffers {
+    float *positions;
+    float *velocities;
+    float *densities;
+    float *accelerations; /* NULL: accelerations are not handed over */
+    void *ready;          /* hipEvent_t; NULL in opened buffers */
+    uint32_t n_particles;
+    uint32_t opened;      /* 1: mapped from another process by vfs_render_buffers_open */
+} vfs_render_buffers;
+
+/* Handles of a set of render buffers for another process (64-byte HIP IPC
+ * handles of the four buffers); plain bytes, pass them over any channel.
+ * Every buffer must be smaller than 2 GiB (n_particles below 178,956,971). */
+typedef struct vfs_render_ipc {
+    unsigned char buffers[4][64];
+    uint32_t n_particles;
+    uint32_t has_accelerations;
+} vfs_render_ipc;
+
+/* SPHModel::CreateDataBuffers (src/models/model.cpp:149-156), device memory
+ * of the current device. */
+int vfs_render_buffers_create(uint32_t n_particles, int with_accelerations, vfs_render_buffers *out);
+/* Frees created buffers, or unmaps opened ones. */
+int vfs_render_buffers_destroy(vfs_render_buffers *buffers);
+/* SPHModel::CopyDataBuffers (src/models/model.cpp:158-170): the state after
+ * the work queued so far, converted to the render layout, ordered on `stream`
+ * like vfs_sim_step; then records buffers->ready.  Accelerations need
+ * VFS_FLAG_STORE_ACCEL when buffers->accelerations is set. */
+int vfs_sim_copy_data_buffers(vfs_sim *sim, const vfs_render_buffers *buffers, void *stream);
+/* Handles of created buffers for a renderer in another process. */
+int vfs_render_buffers_export(const vfs_render_buffers *buffers, vfs_render_ipc *out);
+/* In the renderer's process: maps the buffers of `handles` (read them only). */
+int vfs_render_buffers_open(const vfs_render_ipc *handles, vfs_render_buffers *out);
+/* Makes `stream` wait for the last copy into created buffers. */
+int vfs_render_buffers_wait(const vfs_render_buffers *buffers, void *stream);
+/* Blocks until the last copy into created buffers has finished (no-op for
+ * opened buffers). */
+int vfs_render_buffers_synchronize(const vfs_render_buffers *buffers);
+/* Host copies of the buffers (any pointer may be NULL), after the last copy. */
+int vfs_render_buffers_read(const vfs_render_buffers *buffers, float *positions, float *velocities,
+                            float *densities, float *accelerations);
+
 /* ---- scenes (src/util/scene_loader.cpp:113-140, src/scenes/generic_scene.cpp) */
 typedef struct vfs_scene vfs_scene;
 /* Parses a scene JSON (reference schema + optional extension keys) and builds

@@ -98,3 +98,25 @@ def test_cpp_facade_example_links():
         subprocess.run(["make", "-C", str(exe.parent)], check=True, capture_output=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_render_buffer_argument_checks():
+    assert C.sizeof(V._lib.RenderBuffers) == 48 and C.sizeof(V._lib.RenderIpc) == 264
+    rc, msg = _err(V.lib.vfs_render_buffers_create, 10, 0, None)
+    assert rc == -1 and "null" in msg
+    rc, msg = _err(V.lib.vfs_render_buffers_create, 1 << 28, 0, C.byref(V._lib.RenderBuffers()))
+    assert rc == -1 and "2^28" in msg
+    rc, msg = _err(V.lib.vfs_sim_copy_data_buffers, None, None, None)
+    assert rc == -1 and "null" in msg
+    rc, msg = _err(V.lib.vfs_render_buffers_export, None, None)
+    assert rc == -1 and "null" in msg
+    rc, msg = _err(V.lib.vfs_render_buffers_open, None, None)
+    assert rc == -1 and "null" in msg
+    b = V._lib.RenderBuffers()
+    rc, msg = _err(V.lib.vfs_render_buffers_export, C.byref(b), C.byref(V._lib.RenderIpc()))
+    assert rc == -1 and "incomplete" in msg
+    b.positions = b.velocities = b.densities = 1
+    b.n_particles = 200_000_000
+    rc, msg = _err(V.lib.vfs_render_buffers_export, C.byref(b), C.byref(V._lib.RenderIpc()))
+    assert rc == -1 and "2 GiB" in msg
+    assert V.lib.vfs_render_buffers_destroy(None) == 0

@@ -141,6 +141,60 @@ class Trajectory:
         self.close()
 
 
+class RenderBuffers:
+    """Render buffers in the reference's data-buffer layout (vfs_render_*):
+    packed float3 positions, velocities (and accelerations), float densities.
+    Fill them with copy_from(sim); hand them to another process with
+    export() -> bytes and RenderBuffers.open(bytes) there."""
+
+    def __init__(self, n_particles: int = 0, with_accelerations: bool = False, _opened=None):
+        self.b = _lib.RenderBuffers()
+        if _opened is not None:
+            call("vfs_render_buffers_open", C.byref(_opened), C.byref(self.b))
+        else:
+            call("vfs_render_buffers_create", int(n_particles), int(bool(with_accelerations)), C.byref(self.b))
+        self.n = int(self.b.n_particles)
+
+    @classmethod
+    def open(cls, handles: bytes) -> "RenderBuffers":
+        ipc = _lib.RenderIpc.from_buffer_copy(handles)
+        return cls(_opened=ipc)
+
+    def copy_from(self, sim, stream=None) -> None:
+        call("vfs_sim_copy_data_buffers", sim.handle, C.byref(self.b), stream)
+
+    def export(self) -> bytes:
+        ipc = _lib.RenderIpc()
+        call("vfs_render_buffers_export", C.byref(self.b), C.byref(ipc))
+        return bytes(ipc)
+
+    def wait(self, stream=None) -> None:
+        call("vfs_render_buffers_wait", C.byref(self.b), stream)
+
+    def synchronize(self) -> None:
+        call("vfs_render_buffers_synchronize", C.byref(self.b))
+
+    def read(self):
+        """(positions (n, 3), velocities (n, 3), densities (n,), accelerations (n, 3) or None)."""
+        x = np.empty((self.n, 3), np.float32)
+        v = np.empty((self.n, 3), np.float32)
+        rho = np.empty(self.n, np.float32)
+        a = np.empty((self.n, 3), np.float32) if self.b.accelerations else None
+        call("vfs_render_buffers_read", C.byref(self.b), x.ctypes.data, v.ctypes.data, rho.ctypes.data,
+             a.ctypes.data if a is not None else None)
+        return x, v, rho, a
+
+    def close(self) -> None:
+        if self.b.positions:
+            call("vfs_render_buffers_destroy", C.byref(self.b))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def read_trajectory(path: str) -> dict:
     """A trajectory file as numpy arrays: header fields, objects (k, 16),
     substep (f,), time (f,) and per field (f, n, 3) or (f, n) arrays."""

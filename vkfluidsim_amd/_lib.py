@@ -102,6 +102,26 @@ class DeviceBuffers(C.Structure):
     ]
 
 
+class RenderBuffers(C.Structure):
+    _fields_ = [
+        ("positions", C.c_void_p),
+        ("velocities", C.c_void_p),
+        ("densities", C.c_void_p),
+        ("accelerations", C.c_void_p),
+        ("ready", C.c_void_p),
+        ("n_particles", C.c_uint32),
+        ("opened", C.c_uint32),
+    ]
+
+
+class RenderIpc(C.Structure):
+    _fields_ = [
+        ("buffers", (C.c_ubyte * 64) * 4),
+        ("n_particles", C.c_uint32),
+        ("has_accelerations", C.c_uint32),
+    ]
+
+
 assert C.sizeof(Params) == 84
 assert C.sizeof(BoundaryObject) == 184
 
@@ -186,6 +206,14 @@ SIGNATURES = {
     "vfs_trajectory_open": (_i32, [_vp, C.c_char_p, _u32, C.POINTER(_vp)]),
     "vfs_trajectory_record": (_i32, [_vp, C.c_uint64, C.c_double]),
     "vfs_trajectory_close": (_i32, [_vp]),
+    "vfs_render_buffers_create": (_i32, [_u32, C.c_int, C.POINTER(RenderBuffers)]),
+    "vfs_render_buffers_destroy": (_i32, [C.POINTER(RenderBuffers)]),
+    "vfs_sim_copy_data_buffers": (_i32, [_vp, C.POINTER(RenderBuffers), _vp]),
+    "vfs_render_buffers_export": (_i32, [C.POINTER(RenderBuffers), C.POINTER(RenderIpc)]),
+    "vfs_render_buffers_open": (_i32, [C.POINTER(RenderIpc), C.POINTER(RenderBuffers)]),
+    "vfs_render_buffers_wait": (_i32, [C.POINTER(RenderBuffers), _vp]),
+    "vfs_render_buffers_synchronize": (_i32, [C.POINTER(RenderBuffers)]),
+    "vfs_render_buffers_read": (_i32, [C.POINTER(RenderBuffers), _vp, _vp, _vp, _vp]),
     "vfs_lague_create": (_i32, [C.POINTER(Params), C.POINTER(LagueParams), C.POINTER(_vp)]),
     "vfs_lague_set_params": (_i32, [_vp, C.POINTER(LagueParams)]),
     "vfs_lague_get_params": (_i32, [_vp, C.POINTER(LagueParams)]),
