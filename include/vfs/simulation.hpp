@@ -68,6 +68,15 @@ public:
     void SetParticleState(const std::vector<vec3>& pos, const std::vector<vec3>& vel, uint32_t offset = 0,
                           int64_t count = -1);
 
+    // SPHModel::GetDataBuffers (model.h:38-43): device addresses of the live state.
+    vfs_device_buffers GetDataBuffers() const;
+    // SPHModel::CreateDataBuffers (model.cpp:149-156): render buffers in the
+    // reference's layout; release them with vfs_render_buffers_destroy.
+    vfs_render_buffers CreateDataBuffers(bool with_accelerations = false) const;
+    // SPHModel::CopyDataBuffers (model.cpp:158-170): the state after the work
+    // queued so far into `dst`, ordered on `stream`.
+    void CopyDataBuffers(hipStream_t stream, const vfs_render_buffers& dst) const;
+
     vfs_sim* Handle() const { return sim; }
 
 protected:

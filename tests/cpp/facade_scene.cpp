@@ -1,7 +1,9 @@
 // Drives the C++ facade (include/vfs/simulation.hpp) the way the reference's
 // application drives its classes (src/main.cpp, src/simulation.cpp):
-// LoadScene -> Simulation::SetScene (Init) -> Step once per frame.  Writes the
-// final state to a binary file for the GPU test to compare with the C ABI:
+// LoadScene -> Simulation::SetScene (Init) -> Step once per frame, then
+// CopyDataBuffers into render buffers as SceneRenderer::Draw does
+// (scene_renderer.cpp:52-55).  Writes the final state, read from the render
+// buffers, to a binary file for the GPU test to compare with the C ABI:
 //   u32 n, then n*3 f32 positions, n*3 f32 velocities, n f32 densities, n u32 ids.
 // Usage: facade_scene SCENE.json FRAMES OUT.bin
 #include <cstdio>
@@ -20,8 +22,13 @@ int main(int argc, char** argv) {
         simulation.Init();
         simulation.SetScene(vfs::LoadScene(argv[1]));
         const int frames = std::atoi(argv[2]);
-        for (int f = 0; f < frames; f++) simulation.Step(nullptr);
-        vfs_sim* sim = simulation.GetScene()->GetModel()->Handle();
+        vfs::SPHModel* model = simulation.GetScene()->GetModel();
+        vfs_render_buffers render = model->CreateDataBuffers();
+        for (int f = 0; f < frames; f++) {
+            simulation.Step(nullptr);
+            model->CopyDataBuffers(nullptr, render);
+        }
+        vfs_sim* sim = model->Handle();
         vfs::Check(vfs_sim_synchronize(sim), "vfs_sim_synchronize");
         vfs_params p{};
         vfs::Check(vfs_sim_get_params(sim, &p), "vfs_sim_get_params");
@@ -29,8 +36,10 @@ int main(int argc, char** argv) {
         std::vector<float> x(3 * (size_t)n), v(3 * (size_t)n), rho(n);
         std::vector<uint32_t> ids(n);
         const bool track = (p.flags & VFS_FLAG_TRACK_IDS) != 0;
-        vfs::Check(vfs_sim_get_state(sim, x.data(), v.data(), rho.data(), track ? ids.data() : nullptr),
-                   "vfs_sim_get_state");
+        vfs::Check(vfs_render_buffers_read(&render, x.data(), v.data(), rho.data(), nullptr),
+                   "vfs_render_buffers_read");
+        if (track) vfs::Check(vfs_sim_get_state(sim, nullptr, nullptr, nullptr, ids.data()), "vfs_sim_get_state");
+        vfs::Check(vfs_render_buffers_destroy(&render), "vfs_render_buffers_destroy");
         FILE* out = std::fopen(argv[3], "wb");
         if (!out) {
             std::fprintf(stderr, "cannot write %s\n", argv[3]);

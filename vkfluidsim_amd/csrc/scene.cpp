@@ -264,6 +264,23 @@ void SPHModel::SetParticleState(const std::vector<vec3>& pos, const std::vector<
           "vfs_sim_set_state");
 }
 
+vfs_device_buffers SPHModel::GetDataBuffers() const {
+    vfs_device_buffers b{};
+    Check(vfs_sim_device_buffers(sim, &b), "vfs_sim_device_buffers");
+    return b;
+}
+
+vfs_render_buffers SPHModel::CreateDataBuffers(bool with_accelerations) const {
+    vfs_render_buffers b{};
+    Check(vfs_render_buffers_create((uint32_t)parameters.n_particles, with_accelerations ? 1 : 0, &b),
+          "vfs_render_buffers_create");
+    return b;
+}
+
+void SPHModel::CopyDataBuffers(hipStream_t stream, const vfs_render_buffers& dst) const {
+    Check(vfs_sim_copy_data_buffers(sim, &dst, stream), "vfs_sim_copy_data_buffers");
+}
+
 // SpawnCompactParticlesInBox (model.cpp:35-66) via SetParticlesInBox (:117-137).
 void SPHModel::SetParticlesInBox(const BoundingBox& box, uint32_t offset, int64_t count) {
     if (count == 0) return;
