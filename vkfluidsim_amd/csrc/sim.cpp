@@ -438,8 +438,11 @@ int get_hash_impl(vfs_sim* sim, uint32_t scene, uint32_t* sorted_keys, uint32_t*
         for (uint32_t i = 0; i < n; i++) sorted_indices[i] -= sc.base;
     }
     if (offsets) {
-        VFS_HIP(hipMemcpy(offsets, sim->offsets + sc.base, (size_t)n * 4, hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < n; i++) offsets[i] = offsets[i] == sim->n ? n : offsets[i] - sc.base;
+        // the reference's offsets table (first index of each key, n if empty)
+        // from the bucket starts
+        std::vector<uint32_t> cs((size_t)n + 1);
+        VFS_HIP(hipMemcpy(cs.data(), sim->cell_start + sc.base, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; i++) offsets[i] = cs[i] < cs[i + 1] ? cs[i] - sc.base : n;
     }
     return 0;
 }

@@ -115,14 +115,29 @@ struct vfs_sim {
         return 0;
     }
 
-    // Stable sort of this substep's keys (the preparation kernel also resets
-    // the offsets table to n): a re-sort against the previous substep's
-    // sorted keys once there are any.
-    int sort_substep(hipStream_t s, StageTimer* timer) {
-        if (resort_ready)
-            return resort_keys_iota(plan, keys, sorted_keys, perm, cell_start, sort_ws, s, timer, offsets,
-                                    (size_t)n + 1, n);
-        return sort_keys_iota(plan, keys, sorted_keys, perm, sort_ws, s, timer, offsets, (size_t)n + 1, n);
+    // Stable sort of this substep's keys, then the reorder and the bucket
+    // starts.  Once a previous substep's sorted keys exist the keys are
+    // re-sorted against them and the bucket starts updated from theirs;
+    // otherwise a full sort (whose preparation kernel resets the offsets
+    // table to n), the reorder's key starts and the reverse min-scan.
+    int sort_reorder_ranges(const StepBuffers& b, hipStream_t s, StageTimer* timer) {
+        if (resort_ready) {
+            VFS_TRY(resort_keys_iota(plan, keys, sorted_keys, perm, cell_start, sort_ws, s, timer));
+            StepBuffers bo = b;
+            bo.offsets = nullptr;
+            if (timer) timer->begin(s, "reorder");
+            VFS_TRY(launch_reorder_state(bo, s));
+            if (timer) timer->end(s);
+            return 0;
+        }
+        VFS_TRY(sort_keys_iota(plan, keys, sorted_keys, perm, sort_ws, s, timer, offsets, (size_t)n + 1, n));
+        if (timer) timer->begin(s, "reorder");
+        VFS_TRY(launch_reorder_state(b, s));  // + key starts into the offsets table
+        if (timer) timer->end(s);
+        if (timer) timer->begin(s, "cell_ranges");
+        VFS_TRY(launch_cell_start_scan(plan, offsets, cell_start, sort_ws, s));
+        if (timer) timer->end(s);
+        return 0;
     }
 
     // LagueModel::Step body (lague_model.cpp:106-122) for one substep.
@@ -132,13 +147,7 @@ struct vfs_sim {
         if (timer) timer->begin(s, "predict");
         VFS_TRY(launch_lague_predict(b, c, lague, s));
         if (timer) timer->end(s);
-        VFS_TRY(sort_substep(s, timer));
-        if (timer) timer->begin(s, "reorder");
-        VFS_TRY(launch_reorder_state(b, s));
-        if (timer) timer->end(s);
-        if (timer) timer->begin(s, "cell_ranges");
-        VFS_TRY(launch_cell_start_scan(plan, offsets, cell_start, sort_ws, s));
-        if (timer) timer->end(s);
+        VFS_TRY(sort_reorder_ranges(b, s, timer));
         if (timer) timer->begin(s, "density");
         VFS_TRY(launch_lague_density(b, c, lague, s));
         if (timer) timer->end(s);
@@ -153,13 +162,7 @@ struct vfs_sim {
         const StepBuffers b = buffers();
         const SceneTable table{consts_dev, wave_scene};
         const SceneTable* tp = batch ? &table : nullptr;
-        VFS_TRY(sort_substep(s, timer));
-        if (timer) timer->begin(s, "reorder");
-        VFS_TRY(launch_reorder_state(b, s));  // + key starts into the offsets table
-        if (timer) timer->end(s);
-        if (timer) timer->begin(s, "cell_ranges");
-        VFS_TRY(launch_cell_start_scan(plan, offsets, cell_start, sort_ws, s));
-        if (timer) timer->end(s);
+        VFS_TRY(sort_reorder_ranges(b, s, timer));
         if (timer) timer->begin(s, "density");
         VFS_TRY(launch_density(b, scenes[0].consts, tp, objs_dev, tuning.block_size, tuning.stage_rows, s));
         if (timer) timer->end(s);

@@ -578,6 +578,28 @@ __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict
     }
 }
 
+// Bucket starts of the re-sorted keys, in place over the previous ones: the
+// entries with a key below k are the unchanged entries at previous indices
+// below cs[k] (cs[k] - nb[cs[k]] of them) and the changed keys below k (a
+// lower bound in the sorted changed keys).  m = table size + 1 entries.
+__global__ __launch_bounds__(256) void k_resort_cell_starts(uint32_t* __restrict__ cs, const uint32_t* __restrict__ nb,
+                                                            const uint32_t* __restrict__ bks,
+                                                            const uint32_t* __restrict__ state, uint32_t m) {
+    const uint32_t k = blockIdx.x * (uint32_t)kThreads + threadIdx.x;
+    if (k >= m) return;
+    const uint32_t count = state[0];
+    const uint32_t c = cs[k];
+    uint32_t lo = 0, hi = count;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bks[mid] < k)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    cs[k] = (c - nb[c]) + lo;
+}
+
 inline uint32_t grid_for(uint64_t n, uint32_t block = 256) {
     return (uint32_t)((n + block - 1) / block);
 }
@@ -711,7 +733,7 @@ static int radix_sort(const SortPlan& p, const uint32_t* keys_in, const uint32_t
 // sort of all keys, or (cell_start != nullptr) the re-sort against the
 // previous sorted keys in keys_out and their bucket starts.
 static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
-                     const uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer,
+                     uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer,
                      uint32_t* fill, size_t fill_words, uint32_t fill_value) {
     const uint32_t n = p.n;
     if (n == 0) return 0;
@@ -744,6 +766,11 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
                        cell_start, keys_out, perm_out);
     VFS_TRY(check_launch("sort_merge"));
     if (timer) timer->end(s);
+    if (timer) timer->begin(s, "cell_ranges");
+    hipLaunchKernelGGL(k_resort_cell_starts, dim3(grid_for((uint64_t)n + 1)), dim3(256), 0, s, cell_start, nb, bks,
+                       state, n + 1u);
+    VFS_TRY(check_launch("cell_ranges"));
+    if (timer) timer->end(s);
     return 0;
 }
 
@@ -754,9 +781,8 @@ int sort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_ou
 }
 
 int resort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
-                     const uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer,
-                     uint32_t* fill, size_t fill_words, uint32_t fill_value) {
-    return sort_impl(p, keys_in, keys_out, perm_out, cell_start, workspace, s, timer, fill, fill_words, fill_value);
+                     uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer) {
+    return sort_impl(p, keys_in, keys_out, perm_out, cell_start, workspace, s, timer, nullptr, 0, 0);
 }
 
 int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,

@@ -53,10 +53,11 @@ int sort_keys_iota(const SortPlan& plan, const uint32_t* keys_in, uint32_t* keys
 // The same stable sort when keys_out holds an ascending key array (the
 // previous substep's sorted keys) and cell_start its bucket starts: keys
 // equal to the previous key at their index keep their relative order, the
-// changed ones are radix-sorted on their own (device-side count) and merged.
+// changed ones are radix-sorted on their own (device-side count) and merged;
+// cell_start is then updated in place to the bucket starts of the new keys
+// (plan.n + 1 entries; no offsets table is written).
 int resort_keys_iota(const SortPlan& plan, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
-                     const uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer,
-                     uint32_t* fill = nullptr, size_t fill_words = 0, uint32_t fill_value = 0);
+                     uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer);
 
 // keys[i] = key_base + HashCell3D(floor(pos_i / cell)) % table_size
 int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,

@@ -145,7 +145,8 @@ __device__ __forceinline__ void walk_range(uint32_t jb, uint32_t je, Load&& load
 }
 
 // Reorder (reorder.slang:11-19) of x, v (and ids) by the sort permutation, and
-// CalculateOffsets (spatial_offsets.slang:19-31) into the offsets table.
+// CalculateOffsets (spatial_offsets.slang:19-31) into the offsets table when
+// one is given.
 __global__ __launch_bounds__(256) void k_reorder_state(StepBuffers b, uint32_t nblocks) {
     const uint32_t i = xcd_block(nblocks) * 256u + threadIdx.x;
     if (i >= b.n) return;
@@ -154,6 +155,7 @@ __global__ __launch_bounds__(256) void k_reorder_state(StepBuffers b, uint32_t n
     b.vs[i] = b.v[s];
     if (b.pid) b.pids[i] = b.pid[s];
     if (b.ps) b.ps[i] = b.pred[s];
+    if (!b.offsets) return;
     const uint32_t k = b.sorted_keys[i];
     if (i == 0 || b.sorted_keys[i - 1] != k) b.offsets[k] = i;
 }
