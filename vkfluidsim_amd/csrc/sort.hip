@@ -462,13 +462,17 @@ __global__ __launch_bounds__(256) void k_scan_lookback(uint32_t* __restrict__ da
 //                 [L, U) the previous bucket of key_b (previous cell starts)
 // which is exactly the stable sort, for any number of changed keys.
 
-// Changed keys per tile.
+// Changed keys per tile.  Block 0 also clears the digit histograms and tile
+// counters of the radix passes that sort the changed keys.
 template <int ITEMS>
 __global__ __launch_bounds__(256) void k_resort_count(const uint32_t* __restrict__ keys,
                                                       const uint32_t* __restrict__ old_keys, uint32_t n,
-                                                      uint32_t* __restrict__ tile_counts) {
+                                                      uint32_t* __restrict__ tile_counts,
+                                                      uint32_t* __restrict__ clear, uint32_t clear_words) {
     constexpr int TILE = kThreads * ITEMS;
     __shared__ uint32_t s_tmp[kWaves];
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < clear_words; i += kThreads) clear[i] = 0u;
     const uint32_t base = blockIdx.x * (uint32_t)TILE;
     uint32_t c = 0;
     for (int i = threadIdx.x; i < TILE; i += kThreads) {
@@ -482,17 +486,28 @@ __global__ __launch_bounds__(256) void k_resort_count(const uint32_t* __restrict
 
 // Exclusive count nb of changed keys before each index (tile prefix from the
 // tile counts, then a block scan), the changed keys and indices compacted in
-// index order, and on the last tile the total in state[0] and nb[n].
+// index order, and on the last tile the total in state[0] and nb[n].  For the
+// radix passes over the compacted keys it adds their digit histograms
+// (`passes` digits of `bits` bits) to hist and clears the look-back status of
+// the pass tiles (radix_tile keys each, status rows of `radix` words per
+// tile, pass_words per pass) that its changed keys fall in; later tiles
+// leave without reading any status.
 template <int ITEMS>
 __global__ __launch_bounds__(256) void k_resort_compact(const uint32_t* __restrict__ keys,
                                                         const uint32_t* __restrict__ old_keys, uint32_t n,
                                                         const uint32_t* __restrict__ tile_counts,
                                                         uint32_t* __restrict__ nb, uint32_t* __restrict__ bkeys,
-                                                        uint32_t* __restrict__ bvals, uint32_t* __restrict__ state) {
+                                                        uint32_t* __restrict__ bvals, uint32_t* __restrict__ state,
+                                                        uint32_t* __restrict__ hist, int passes, int bits,
+                                                        uint32_t* __restrict__ status, uint32_t radix_tile,
+                                                        size_t pass_words) {
     constexpr int TILE = kThreads * ITEMS;
     __shared__ uint8_t s_flag[TILE];
     __shared__ uint32_t s_data[TILE];
     __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_hist[kMaxPasses << 10];
+    const uint32_t radix = 1u << bits, mask = radix - 1u;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)passes * radix; i += kThreads) s_hist[i] = 0u;
     const int tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)TILE;
@@ -526,13 +541,27 @@ __global__ __launch_bounds__(256) void k_resort_compact(const uint32_t* __restri
         const uint32_t e = s_data[i];
         nb[idx] = e;
         if (s_flag[i]) {
-            bkeys[e] = keys[idx];
+            const uint32_t k = keys[idx];
+            bkeys[e] = k;
             bvals[e] = idx;
+            for (int ps = 0; ps < passes; ps++) atomicAdd(&s_hist[ps * radix + ((k >> (ps * bits)) & mask)], 1u);
         }
+    }
+    if (total) {
+        // status rows of the pass tiles holding changed keys prefix .. prefix + total - 1
+        const uint32_t t0 = prefix / radix_tile, t1 = (prefix + total - 1u) / radix_tile;
+        const uint32_t words = (t1 - t0 + 1u) * radix;
+        for (int ps = 0; ps < passes; ps++)
+            for (uint32_t i = tid; i < words; i += kThreads) status[ps * pass_words + (size_t)t0 * radix + i] = 0u;
     }
     if (tile == gridDim.x - 1u && tid == 0) {
         nb[n] = prefix + total;
         state[0] = prefix + total;
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < (uint32_t)passes * radix; i += kThreads) {
+        const uint32_t c = s_hist[i];
+        if (c) atomicAdd(&hist[i], c);
     }
 }
 
@@ -687,12 +716,13 @@ static void launch_pass_items(int items, const uint32_t* ki, const uint32_t* vi,
 // n_dev: element count read on the device (at most p.n).
 static int radix_sort(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                       uint32_t* vals_out, const uint32_t* n_dev, uint32_t* workspace, hipStream_t s,
-                      StageTimer* timer) {
+                      StageTimer* timer, bool have_histograms = false) {
     const uint32_t n = p.n;
     const size_t radix = (size_t)1 << p.bits;
     uint32_t* hist = workspace + p.off_hist;
     uint32_t* counters = workspace + p.off_counters;
     uint32_t* status = workspace + p.off_status;
+    if (!have_histograms) {
     if (timer) timer->begin(s, "sort_histogram");
     const uint32_t hblocks = std::max<uint32_t>(1u, std::min<uint32_t>(grid_for(n, 256u * 16u), 1024u));
     if (p.bits == 8)
@@ -704,6 +734,7 @@ static int radix_sort(const SortPlan& p, const uint32_t* keys_in, const uint32_t
                            n_dev);
     VFS_TRY(check_launch("sort_histogram"));
     if (timer) timer->end(s);
+    }
     uint32_t* tmpk[2] = {workspace + p.off_tmp[0], workspace + p.off_tmp[2]};
     uint32_t* tmpv[2] = {workspace + p.off_tmp[1], workspace + p.off_tmp[3]};
     for (int pass = 0; pass < p.passes; pass++) {
@@ -737,14 +768,18 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
                      uint32_t* fill, size_t fill_words, uint32_t fill_value) {
     const uint32_t n = p.n;
     if (n == 0) return 0;
-    const size_t zero_words = p.workspace_words - p.zero_begin;
-    if (timer) timer->begin(s, "sort_prepare");
-    hipLaunchKernelGGL(k_prepare, dim3(std::min<uint32_t>(grid_for(std::max(zero_words, fill_words)), 2048u)),
-                       dim3(256), 0, s, workspace + p.zero_begin, zero_words, fill, fill ? fill_words : 0,
-                       fill_value);
-    VFS_TRY(check_launch("sort_prepare"));
-    if (timer) timer->end(s);
-    if (!cell_start) return radix_sort(p, keys_in, nullptr, keys_out, perm_out, nullptr, workspace, s, timer);
+    if (!cell_start) {
+        const size_t zero_words = p.workspace_words - p.zero_begin;
+        if (timer) timer->begin(s, "sort_prepare");
+        hipLaunchKernelGGL(k_prepare, dim3(std::min<uint32_t>(grid_for(std::max(zero_words, fill_words)), 2048u)),
+                           dim3(256), 0, s, workspace + p.zero_begin, zero_words, fill, fill ? fill_words : 0,
+                           fill_value);
+        VFS_TRY(check_launch("sort_prepare"));
+        if (timer) timer->end(s);
+        return radix_sort(p, keys_in, nullptr, keys_out, perm_out, nullptr, workspace, s, timer);
+    }
+    // re-sort: the count and compaction kernels clear what the radix passes
+    // over the changed keys need and build their histograms
 
     uint32_t* nb = workspace + p.off_rs_nb;
     uint32_t* bkeys = workspace + p.off_rs_bkeys;
@@ -754,13 +789,17 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
     uint32_t* counts = workspace + p.off_rs_tile_counts;
     uint32_t* state = workspace + p.off_rs_state;
     if (timer) timer->begin(s, "sort_scan");
-    hipLaunchKernelGGL(k_resort_count<kResortItems>, dim3(p.rs_tiles), dim3(256), 0, s, keys_in, keys_out, n, counts);
+    const uint32_t clear_words = (uint32_t)(p.off_counters + kMaxPasses - p.off_hist);  // histograms + tile counters
+    hipLaunchKernelGGL(k_resort_count<kResortItems>, dim3(p.rs_tiles), dim3(256), 0, s, keys_in, keys_out, n, counts,
+                       workspace + p.off_hist, clear_words);
     VFS_TRY(check_launch("sort_scan"));
+    const size_t radix = (size_t)1 << p.bits;
     hipLaunchKernelGGL(k_resort_compact<kResortItems>, dim3(p.rs_tiles), dim3(256), 0, s, keys_in, keys_out, n,
-                       counts, nb, bkeys, bvals, state);
+                       counts, nb, bkeys, bvals, state, workspace + p.off_hist, p.passes, p.bits,
+                       workspace + p.off_status, (uint32_t)(kThreads * p.items), (size_t)p.num_tiles * radix);
     VFS_TRY(check_launch("sort_scan"));
     if (timer) timer->end(s);
-    VFS_TRY(radix_sort(p, bkeys, bvals, bks, bis, state, workspace, s, timer));
+    VFS_TRY(radix_sort(p, bkeys, bvals, bks, bis, state, workspace, s, timer, true));
     if (timer) timer->begin(s, "sort_merge");
     hipLaunchKernelGGL(k_resort_merge, dim3(grid_for(n)), dim3(256), 0, s, keys_in, nb, n, state, bks, bis,
                        cell_start, keys_out, perm_out);
