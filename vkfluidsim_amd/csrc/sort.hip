@@ -723,17 +723,19 @@ static int radix_sort(const SortPlan& p, const uint32_t* keys_in, const uint32_t
     uint32_t* counters = workspace + p.off_counters;
     uint32_t* status = workspace + p.off_status;
     if (!have_histograms) {
-    if (timer) timer->begin(s, "sort_histogram");
-    const uint32_t hblocks = std::max<uint32_t>(1u, std::min<uint32_t>(grid_for(n, 256u * 16u), 1024u));
-    if (p.bits == 8)
-        hipLaunchKernelGGL(k_radix_histogram<8>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist, n_dev);
-    else if (p.bits == 9)
-        hipLaunchKernelGGL(k_radix_histogram<9>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist, n_dev);
-    else
-        hipLaunchKernelGGL(k_radix_histogram<10>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist,
-                           n_dev);
-    VFS_TRY(check_launch("sort_histogram"));
-    if (timer) timer->end(s);
+        if (timer) timer->begin(s, "sort_histogram");
+        const uint32_t hblocks = std::max<uint32_t>(1u, std::min<uint32_t>(grid_for(n, 256u * 16u), 1024u));
+        if (p.bits == 8)
+            hipLaunchKernelGGL(k_radix_histogram<8>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist,
+                               n_dev);
+        else if (p.bits == 9)
+            hipLaunchKernelGGL(k_radix_histogram<9>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist,
+                               n_dev);
+        else
+            hipLaunchKernelGGL(k_radix_histogram<10>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist,
+                               n_dev);
+        VFS_TRY(check_launch("sort_histogram"));
+        if (timer) timer->end(s);
     }
     uint32_t* tmpk[2] = {workspace + p.off_tmp[0], workspace + p.off_tmp[2]};
     uint32_t* tmpv[2] = {workspace + p.off_tmp[1], workspace + p.off_tmp[3]};
