@@ -179,8 +179,9 @@ int vfs_sim_get_tuning(const vfs_sim *sim, vfs_tuning *tuning);
 int vfs_sim_set_tuning(vfs_sim *sim, const vfs_tuning *tuning);
 /* Autotuner: times every vfs_tuning on `substeps_per_trial` substeps from the
  * current state (restored before each trial and at the end, so the scene does
- * not advance), keeps the fastest and returns it with its mean ms per substep.  No counterpart in the reference (its launch shape
- * is fixed: 256 threads, atomic counting sort). */
+ * not advance), keeps the fastest and returns it with its mean ms per
+ * substep.  No counterpart in the reference (its launch shape is fixed: 256
+ * threads, atomic counting sort). */
 int vfs_sim_autotune(vfs_sim *sim, uint32_t substeps_per_trial, vfs_tuning *best, float *best_ms_per_substep);
 /* Runs n substeps eagerly (advancing the state) with a HIP event pair around
  * every kernel on the simulation stream.  For each stage writes the mean time

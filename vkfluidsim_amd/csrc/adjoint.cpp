@@ -262,8 +262,8 @@ int vfs_adjoint_backward(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev,
         for (int k = (int)steps - 1; k >= 0; k--) {
             const auto& t = a->tape[k];
             VFS_TRY(launch_adjoint_substep(t.perm, t.xs, t.vs, t.rho, t.bnd, t.nbr, t.cnt, gx_cur, gv_cur, a->abar,
-                                           a->gx, a->gv, a->rbar, a->gpsi, a->dup, gx_nxt, gv_nxt, a->nu_part, a->nu_bar, n, c,
-                                           sim->objs_dev, s));
+                                           a->gx, a->gv, a->rbar, a->gpsi, a->dup, gx_nxt, gv_nxt, a->nu_part,
+                                           a->nu_bar, n, c, sim->objs_dev, s));
             std::swap(gx_cur, gx_nxt);
             std::swap(gv_cur, gv_nxt);
         }

@@ -574,7 +574,8 @@ __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ bks,
                                                       const uint32_t* __restrict__ bis,
                                                       const uint32_t* __restrict__ cell_start,
-                                                      uint32_t* __restrict__ keys_out, uint32_t* __restrict__ perm_out) {
+                                                      uint32_t* __restrict__ keys_out,
+                                                      uint32_t* __restrict__ perm_out) {
     const uint32_t g = blockIdx.x * (uint32_t)kThreads + threadIdx.x;
     if (g >= n) return;
     const uint32_t count = state[0];

@@ -9,11 +9,12 @@
 // with -ffp-contract=off so that every float operation rounds once, as in the
 // CPU oracle; a vector divided by a scalar, and r / h in the smoothing kernels,
 // are multiplied by the IEEE reciprocal of the divisor, other scalar quotients
-// are IEEE divisions (the oracle's canonical reading of the shader arithmetic).  Neighbour enumeration visits the 27 hashed cells in the
-// reference order (z outer, y, x inner; spatial_hash_3d.slang:3-10) and each
-// bucket in ascending sorted index, so sums are accumulated in the reference
-// order.  When the three x-adjacent cells of a row hash to consecutive keys
-// their buckets are contiguous in sorted order and are walked as one range.
+// are IEEE divisions (the oracle's canonical reading of the shader arithmetic).
+// Neighbour enumeration visits the 27 hashed cells in the reference order (z
+// outer, y, x inner; spatial_hash_3d.slang:3-10) and each bucket in ascending
+// sorted index, so sums are accumulated in the reference order.  When the
+// three x-adjacent cells of a row hash to consecutive keys their buckets are
+// contiguous in sorted order and are walked as one range.
 #include "volume_map.h"
 #include "wcsph_math.h"
 
