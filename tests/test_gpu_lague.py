@@ -119,8 +119,8 @@ def test_lague_profile_stages():
     sim = V.Lague(_params(w))
     sim.set_state(w.x, w.v)
     names = [s[0] for s in sim.profile(3)]
-    assert names == ["predict", "sort_prepare", "sort_scan", "sort_histogram", "sort_pass", "sort_merge", "reorder",
-                     "cell_ranges", "density", "forces_integrate"]
+    assert names == ["predict", "sort_prepare", "sort_scan", "sort_histogram", "sort_small", "sort_pass",
+                     "sort_merge", "reorder", "cell_ranges", "density", "forces_integrate"]
     with pytest.raises(V.VfsError, match="no boundary objects"):
         sim.get_boundary()
 

@@ -230,9 +230,9 @@ def test_profile_stages_report_every_kernel():
     stages = sim.profile(3)
     names = [s[0] for s in stages]
     # the first substep sorts from scratch, the others re-sort (changed keys
-    # collected, radix-sorted on their own, merged)
-    assert names == ["sort_prepare", "sort_scan", "sort_histogram", "sort_pass", "sort_merge", "reorder",
-                     "cell_ranges", "density", "forces_integrate"]
+    # collected, sorted on their own by one block or the radix passes, merged)
+    assert names == ["sort_prepare", "sort_scan", "sort_histogram", "sort_small", "sort_pass", "sort_merge",
+                     "reorder", "cell_ranges", "density", "forces_integrate"]
     assert all(ms > 0 for _, ms, _ in stages)
     assert dict((s[0], s[2]) for s in stages)["sort_pass"] == 2  # 12-bit keys at n = 4096
 

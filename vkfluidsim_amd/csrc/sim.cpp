@@ -564,9 +564,9 @@ int autotune_impl(vfs_sim* sim, uint32_t substeps_per_trial, bool allow_cell_siz
 int profile_impl(vfs_sim* sim, uint32_t n, float* ms, uint32_t* launches, const char** names, uint32_t max_stages,
                  uint32_t* n_stages) {
     VFS_CHECK(sim && n_stages, VFS_ERR_INVALID, "null argument");
-    static const char* kNames[] = {"predict", "sort_prepare", "sort_scan",   "sort_histogram",
-                                   "sort_pass", "sort_merge", "reorder",     "cell_ranges",
-                                   "density", "forces_integrate"};
+    static const char* kNames[] = {"predict",   "sort_prepare", "sort_scan", "sort_histogram", "sort_small",
+                                   "sort_pass", "sort_merge",   "reorder",   "cell_ranges",    "density",
+                                   "forces_integrate"};
     constexpr uint32_t kN = sizeof(kNames) / sizeof(kNames[0]);
     *n_stages = 0;
     if (sim->n == 0 || n == 0) return 0;

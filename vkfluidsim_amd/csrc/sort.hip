@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ hist,
                                                     uint32_t* __restrict__ status,
                                                     uint32_t* __restrict__ tile_counter,
-                                                    const uint32_t* __restrict__ n_dev) {
+                                                    const uint32_t* __restrict__ n_dev, uint32_t small_cap) {
     constexpr int RADIX = 1 << BITS;
     constexpr int DPT = RADIX / kThreads;  // digits per thread
     constexpr int TILE = kThreads * ITEMS;
@@ -126,6 +126,8 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
     __shared__ uint32_t s_vals[TILE];
     __shared__ uint32_t s_tmp[kWaves];
 
+    // a device-side count of at most small_cap keys was sorted by k_resort_small
+    if (n_dev && *n_dev <= small_cap) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     for (int i = tid; i < kWaves * RADIX; i += kThreads) (&s_wave[0][0])[i] = 0;
@@ -241,6 +243,109 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
         const uint32_t dst = (uint32_t)(s_global[d] + (int32_t)i);
         keys_out[dst] = k;
         vals_out[dst] = s_vals[i];
+    }
+}
+
+// The whole stable LSD radix sort of at most kSmallSort (key, value) pairs
+// (count read on the device) in one block's LDS: each pass ranks the keys as
+// k_radix_pass ranks a tile (wave ballots; wave-major, item-major, lane order =
+// input order) and scatters them within LDS.  A larger count leaves at once
+// (the multi-tile passes sort it).
+constexpr int kSmallItems = 16;
+constexpr uint32_t kSmallSort = (uint32_t)kThreads * kSmallItems;
+template <int BITS>
+__global__ __launch_bounds__(256) void k_resort_small(const uint32_t* __restrict__ keys_in,
+                                                      const uint32_t* __restrict__ vals_in,
+                                                      uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+                                                      const uint32_t* __restrict__ n_dev, int passes) {
+    constexpr int RADIX = 1 << BITS;
+    constexpr int DPT = RADIX / kThreads;
+    constexpr int ITEMS = kSmallItems;
+    __shared__ uint32_t s_wave[kWaves][RADIX];
+    __shared__ uint32_t s_local[RADIX];
+    __shared__ uint32_t s_keys[kSmallSort];
+    __shared__ uint32_t s_vals[kSmallSort];
+    __shared__ uint32_t s_tmp[kWaves];
+    const uint32_t n = *n_dev;
+    if (n == 0 || n > kSmallSort) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (uint32_t i = (uint32_t)tid; i < n; i += kThreads) {
+        s_keys[i] = keys_in[i];
+        s_vals[i] = vals_in[i];
+    }
+    const uint64_t lt = lanemask_lt();
+    const uint32_t wave_base = (uint32_t)w * 64u * ITEMS;
+    for (int pass = 0; pass < passes; pass++) {
+        const int shift = pass * BITS;
+        for (int i = tid; i < kWaves * RADIX; i += kThreads) (&s_wave[0][0])[i] = 0;
+        __syncthreads();  // the clear (and, before the first pass, the loads) are done
+        uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const uint32_t idx = wave_base + (uint32_t)it * 64u + (uint32_t)lane;
+            const bool valid = idx < n;
+            key[it] = valid ? s_keys[idx] : 0xFFFFFFFFu;
+            val[it] = valid ? s_vals[idx] : 0u;
+        }
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const uint32_t idx = wave_base + (uint32_t)it * 64u + (uint32_t)lane;
+            const bool valid = idx < n;
+            const uint32_t d = (key[it] >> shift) & (RADIX - 1);
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < BITS; b++) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            if (valid) {
+                const uint32_t before = (uint32_t)__popcll(peers & lt);
+                const uint32_t prior = s_wave[w][d];
+                rank[it] = prior + before;
+                if (before == 0) s_wave[w][d] = prior + (uint32_t)__popcll(peers);
+            }
+        }
+        __syncthreads();
+        // per digit: exclusive offsets across waves, then the digit starts
+        uint32_t tcount[DPT];
+        uint32_t tsum = 0;
+#pragma unroll
+        for (int dd = 0; dd < DPT; dd++) {
+            const int d = tid * DPT + dd;
+            uint32_t c = 0;
+#pragma unroll
+            for (int ww = 0; ww < kWaves; ww++) {
+                const uint32_t x = s_wave[ww][d];
+                s_wave[ww][d] = c;
+                c += x;
+            }
+            tcount[dd] = c;
+            tsum += c;
+        }
+        uint32_t tot;
+        uint32_t lrun = block_exclusive_scan(tsum, s_tmp, tot);
+#pragma unroll
+        for (int dd = 0; dd < DPT; dd++) {
+            s_local[tid * DPT + dd] = lrun;
+            lrun += tcount[dd];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < ITEMS; it++) {
+            const uint32_t idx = wave_base + (uint32_t)it * 64u + (uint32_t)lane;
+            if (idx < n) {
+                const uint32_t d = (key[it] >> shift) & (RADIX - 1);
+                const uint32_t pos = s_local[d] + s_wave[w][d] + rank[it];
+                s_keys[pos] = key[it];
+                s_vals[pos] = val[it];
+            }
+        }
+        __syncthreads();  // the scatter is done before the next pass clears s_wave and reads s_keys
+    }
+    for (uint32_t i = (uint32_t)tid; i < n; i += kThreads) {
+        keys_out[i] = s_keys[i];
+        vals_out[i] = s_vals[i];
     }
 }
 
@@ -696,28 +801,28 @@ int valid_sort_config(int bits, int items) {
 template <int BITS, int ITEMS>
 static void launch_pass(const uint32_t* ki, const uint32_t* vi, uint32_t* ko, uint32_t* vo, uint32_t n, int shift,
                         const uint32_t* hist, uint32_t* status, uint32_t* counter, uint32_t tiles,
-                        const uint32_t* n_dev, hipStream_t s) {
+                        const uint32_t* n_dev, uint32_t small_cap, hipStream_t s) {
     hipLaunchKernelGGL((k_radix_pass<BITS, ITEMS>), dim3(tiles), dim3(kThreads), 0, s, ki, vi, ko, vo, n, shift,
-                       hist, status, counter, n_dev);
+                       hist, status, counter, n_dev, small_cap);
 }
 
 template <int BITS>
 static void launch_pass_items(int items, const uint32_t* ki, const uint32_t* vi, uint32_t* ko, uint32_t* vo,
                               uint32_t n, int shift, const uint32_t* hist, uint32_t* status, uint32_t* counter,
-                              uint32_t tiles, const uint32_t* n_dev, hipStream_t s) {
+                              uint32_t tiles, const uint32_t* n_dev, uint32_t small_cap, hipStream_t s) {
     if (items == 8)
-        launch_pass<BITS, 8>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, n_dev, s);
+        launch_pass<BITS, 8>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, n_dev, small_cap, s);
     else if (items == 12)
-        launch_pass<BITS, 12>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, n_dev, s);
+        launch_pass<BITS, 12>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, n_dev, small_cap, s);
     else
-        launch_pass<BITS, 16>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, n_dev, s);
+        launch_pass<BITS, 16>(ki, vi, ko, vo, n, shift, hist, status, counter, tiles, n_dev, small_cap, s);
 }
 
 // Stable radix sort of (keys_in, vals_in or iota) into keys_out / vals_out;
 // n_dev: element count read on the device (at most p.n).
 static int radix_sort(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
                       uint32_t* vals_out, const uint32_t* n_dev, uint32_t* workspace, hipStream_t s,
-                      StageTimer* timer, bool have_histograms = false) {
+                      StageTimer* timer, bool have_histograms = false, uint32_t small_cap = 0) {
     const uint32_t n = p.n;
     const size_t radix = (size_t)1 << p.bits;
     uint32_t* hist = workspace + p.off_hist;
@@ -750,13 +855,13 @@ static int radix_sort(const SortPlan& p, const uint32_t* keys_in, const uint32_t
         if (timer) timer->begin(s, "sort_pass");
         if (p.bits == 8)
             launch_pass_items<8>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
-                                 counters + pass, p.num_tiles, n_dev, s);
+                                 counters + pass, p.num_tiles, n_dev, small_cap, s);
         else if (p.bits == 9)
             launch_pass_items<9>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
-                                 counters + pass, p.num_tiles, n_dev, s);
+                                 counters + pass, p.num_tiles, n_dev, small_cap, s);
         else
             launch_pass_items<10>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
-                                  counters + pass, p.num_tiles, n_dev, s);
+                                  counters + pass, p.num_tiles, n_dev, small_cap, s);
         VFS_TRY(check_launch("sort_pass"));
         if (timer) timer->end(s);
     }
@@ -802,7 +907,17 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
                        workspace + p.off_status, (uint32_t)(kThreads * p.items), (size_t)p.num_tiles * radix);
     VFS_TRY(check_launch("sort_scan"));
     if (timer) timer->end(s);
-    VFS_TRY(radix_sort(p, bkeys, bvals, bks, bis, state, workspace, s, timer, true));
+    // a few changed keys are sorted by one block in LDS, more by the radix passes
+    if (timer) timer->begin(s, "sort_small");
+    if (p.bits == 8)
+        hipLaunchKernelGGL(k_resort_small<8>, dim3(1), dim3(256), 0, s, bkeys, bvals, bks, bis, state, p.passes);
+    else if (p.bits == 9)
+        hipLaunchKernelGGL(k_resort_small<9>, dim3(1), dim3(256), 0, s, bkeys, bvals, bks, bis, state, p.passes);
+    else
+        hipLaunchKernelGGL(k_resort_small<10>, dim3(1), dim3(256), 0, s, bkeys, bvals, bks, bis, state, p.passes);
+    VFS_TRY(check_launch("sort_small"));
+    if (timer) timer->end(s);
+    VFS_TRY(radix_sort(p, bkeys, bvals, bks, bis, state, workspace, s, timer, true, kSmallSort));
     if (timer) timer->begin(s, "sort_merge");
     hipLaunchKernelGGL(k_resort_merge, dim3(grid_for(n)), dim3(256), 0, s, keys_in, nb, n, state, bks, bis,
                        cell_start, keys_out, perm_out);
