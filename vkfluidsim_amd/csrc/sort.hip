@@ -616,6 +616,9 @@ __global__ __launch_bounds__(256) void k_resort_compact(const uint32_t* __restri
     const int tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)TILE;
+    // the tile's prefix: every block sums the counts of the tiles before it
+    // (tiles^2 / 2 reads of an L2-resident array: 2M at 8M particles; at the
+    // 2^28-particle limit about 0.3% of a substep)
     uint32_t before = 0;
     for (uint32_t t = (uint32_t)tid; t < tile; t += kThreads) before += tile_counts[t];
     uint32_t prefix;
