@@ -781,9 +781,12 @@ SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int i
     p.off_rs_bvals = take(n);
     p.off_rs_bks = take(n);
     p.off_rs_bis = take(n);
-    p.off_hist = take(kMaxPasses * radix);
+    p.rs_bits = 10;
+    p.rs_passes = kb == 0 ? 1 : (kb + p.rs_bits - 1) / p.rs_bits;
+    const size_t rs_radix = (size_t)1 << p.rs_bits;
+    p.off_hist = take(kMaxPasses * std::max(radix, rs_radix));
     p.off_counters = take(kMaxPasses);
-    p.off_status = take((size_t)p.passes * p.num_tiles * radix);
+    p.off_status = take(std::max((size_t)p.passes * radix, (size_t)p.rs_passes * rs_radix) * p.num_tiles);
     const uint32_t cs_tile = (uint32_t)kThreads * kCellStartItems;
     p.cs_tiles = (uint32_t)(((uint64_t)n + 1 + cs_tile - 1) / cs_tile);
     p.off_cs_counter = take(1);
@@ -823,47 +826,47 @@ static void launch_pass_items(int items, const uint32_t* ki, const uint32_t* vi,
 
 // Stable radix sort of (keys_in, vals_in or iota) into keys_out / vals_out;
 // n_dev: element count read on the device (at most p.n).
-static int radix_sort(const SortPlan& p, const uint32_t* keys_in, const uint32_t* vals_in, uint32_t* keys_out,
-                      uint32_t* vals_out, const uint32_t* n_dev, uint32_t* workspace, hipStream_t s,
-                      StageTimer* timer, bool have_histograms = false, uint32_t small_cap = 0) {
+static int radix_sort(const SortPlan& p, int bits, int passes, const uint32_t* keys_in, const uint32_t* vals_in,
+                      uint32_t* keys_out, uint32_t* vals_out, const uint32_t* n_dev, uint32_t* workspace,
+                      hipStream_t s, StageTimer* timer, bool have_histograms = false, uint32_t small_cap = 0) {
     const uint32_t n = p.n;
-    const size_t radix = (size_t)1 << p.bits;
+    const size_t radix = (size_t)1 << bits;
     uint32_t* hist = workspace + p.off_hist;
     uint32_t* counters = workspace + p.off_counters;
     uint32_t* status = workspace + p.off_status;
     if (!have_histograms) {
         if (timer) timer->begin(s, "sort_histogram");
         const uint32_t hblocks = std::max<uint32_t>(1u, std::min<uint32_t>(grid_for(n, 256u * 16u), 1024u));
-        if (p.bits == 8)
-            hipLaunchKernelGGL(k_radix_histogram<8>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist,
+        if (bits == 8)
+            hipLaunchKernelGGL(k_radix_histogram<8>, dim3(hblocks), dim3(256), 0, s, keys_in, n, passes, hist,
                                n_dev);
-        else if (p.bits == 9)
-            hipLaunchKernelGGL(k_radix_histogram<9>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist,
+        else if (bits == 9)
+            hipLaunchKernelGGL(k_radix_histogram<9>, dim3(hblocks), dim3(256), 0, s, keys_in, n, passes, hist,
                                n_dev);
         else
-            hipLaunchKernelGGL(k_radix_histogram<10>, dim3(hblocks), dim3(256), 0, s, keys_in, n, p.passes, hist,
+            hipLaunchKernelGGL(k_radix_histogram<10>, dim3(hblocks), dim3(256), 0, s, keys_in, n, passes, hist,
                                n_dev);
         VFS_TRY(check_launch("sort_histogram"));
         if (timer) timer->end(s);
     }
     uint32_t* tmpk[2] = {workspace + p.off_tmp[0], workspace + p.off_tmp[2]};
     uint32_t* tmpv[2] = {workspace + p.off_tmp[1], workspace + p.off_tmp[3]};
-    for (int pass = 0; pass < p.passes; pass++) {
+    for (int pass = 0; pass < passes; pass++) {
         const uint32_t* ki = pass == 0 ? keys_in : tmpk[(pass - 1) & 1];
         const uint32_t* vi = pass == 0 ? vals_in : tmpv[(pass - 1) & 1];
-        const bool last = pass == p.passes - 1;
+        const bool last = pass == passes - 1;
         uint32_t* ko = last ? keys_out : tmpk[pass & 1];
         uint32_t* vo = last ? vals_out : tmpv[pass & 1];
         uint32_t* st = status + (size_t)pass * p.num_tiles * radix;
         if (timer) timer->begin(s, "sort_pass");
-        if (p.bits == 8)
-            launch_pass_items<8>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
+        if (bits == 8)
+            launch_pass_items<8>(p.items, ki, vi, ko, vo, n, pass * bits, hist + pass * radix, st,
                                  counters + pass, p.num_tiles, n_dev, small_cap, s);
-        else if (p.bits == 9)
-            launch_pass_items<9>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
+        else if (bits == 9)
+            launch_pass_items<9>(p.items, ki, vi, ko, vo, n, pass * bits, hist + pass * radix, st,
                                  counters + pass, p.num_tiles, n_dev, small_cap, s);
         else
-            launch_pass_items<10>(p.items, ki, vi, ko, vo, n, pass * p.bits, hist + pass * radix, st,
+            launch_pass_items<10>(p.items, ki, vi, ko, vo, n, pass * bits, hist + pass * radix, st,
                                   counters + pass, p.num_tiles, n_dev, small_cap, s);
         VFS_TRY(check_launch("sort_pass"));
         if (timer) timer->end(s);
@@ -887,7 +890,7 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
                            fill_value);
         VFS_TRY(check_launch("sort_prepare"));
         if (timer) timer->end(s);
-        return radix_sort(p, keys_in, nullptr, keys_out, perm_out, nullptr, workspace, s, timer);
+        return radix_sort(p, p.bits, p.passes, keys_in, nullptr, keys_out, perm_out, nullptr, workspace, s, timer);
     }
     // re-sort: the count and compaction kernels clear what the radix passes
     // over the changed keys need and build their histograms
@@ -904,23 +907,20 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
     hipLaunchKernelGGL(k_resort_count<kResortItems>, dim3(p.rs_tiles), dim3(256), 0, s, keys_in, keys_out, n, counts,
                        workspace + p.off_hist, clear_words);
     VFS_TRY(check_launch("sort_scan"));
-    const size_t radix = (size_t)1 << p.bits;
+    const size_t rs_radix = (size_t)1 << p.rs_bits;
     hipLaunchKernelGGL(k_resort_compact<kResortItems>, dim3(p.rs_tiles), dim3(256), 0, s, keys_in, keys_out, n,
-                       counts, nb, bkeys, bvals, state, workspace + p.off_hist, p.passes, p.bits,
-                       workspace + p.off_status, (uint32_t)(kThreads * p.items), (size_t)p.num_tiles * radix);
+                       counts, nb, bkeys, bvals, state, workspace + p.off_hist, p.rs_passes, p.rs_bits,
+                       workspace + p.off_status, (uint32_t)(kThreads * p.items), (size_t)p.num_tiles * rs_radix);
     VFS_TRY(check_launch("sort_scan"));
     if (timer) timer->end(s);
     // a few changed keys are sorted by one block in LDS, more by the radix passes
+    static_assert(kMaxPasses * 1024 >= 3 * 1024, "histograms of the 10-bit re-sort digits fit");
     if (timer) timer->begin(s, "sort_small");
-    if (p.bits == 8)
-        hipLaunchKernelGGL(k_resort_small<8>, dim3(1), dim3(256), 0, s, bkeys, bvals, bks, bis, state, p.passes);
-    else if (p.bits == 9)
-        hipLaunchKernelGGL(k_resort_small<9>, dim3(1), dim3(256), 0, s, bkeys, bvals, bks, bis, state, p.passes);
-    else
-        hipLaunchKernelGGL(k_resort_small<10>, dim3(1), dim3(256), 0, s, bkeys, bvals, bks, bis, state, p.passes);
+    hipLaunchKernelGGL(k_resort_small<10>, dim3(1), dim3(256), 0, s, bkeys, bvals, bks, bis, state, p.rs_passes);
     VFS_TRY(check_launch("sort_small"));
     if (timer) timer->end(s);
-    VFS_TRY(radix_sort(p, bkeys, bvals, bks, bis, state, workspace, s, timer, true, kSmallSort));
+    VFS_TRY(radix_sort(p, p.rs_bits, p.rs_passes, bkeys, bvals, bks, bis, state, workspace, s, timer, true,
+                       kSmallSort));
     if (timer) timer->begin(s, "sort_merge");
     hipLaunchKernelGGL(k_resort_merge, dim3(grid_for(n)), dim3(256), 0, s, keys_in, nb, n, state, bks, bis,
                        cell_start, keys_out, perm_out);

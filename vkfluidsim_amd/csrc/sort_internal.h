@@ -31,7 +31,9 @@ struct SortPlan {
     size_t off_hist, off_counters, off_status;
     uint32_t cs_tiles;                  // tiles of the cell-start scan over n + 1 entries
     size_t off_cs_counter, off_cs_status;
-    // re-sort (changed keys sorted and merged into the previous order)
+    // re-sort (changed keys sorted and merged into the previous order); the
+    // changed keys are sorted with rs_bits-wide digits (fewer passes)
+    int rs_bits, rs_passes;
     uint32_t rs_tiles;
     size_t off_rs_nb, off_rs_bkeys, off_rs_bvals, off_rs_bks, off_rs_bis;
     size_t off_rs_tile_counts, off_rs_state;
