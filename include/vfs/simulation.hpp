@@ -123,6 +123,35 @@ private:
     uint32_t flags;
 };
 
+// LagueModel (src/models/lague_model.h:15-45): the Clavet double-density
+// model on predicted positions; default parameters as lague_model.cpp:26-34.
+class LagueModel final : public SPHModel {
+public:
+    struct Parameters {
+        float wall_damping_factor{0.95f};
+        float pressure_multiplier{288.0f};
+        float near_pressure_multiplier{2.16f};
+        float viscosity_strenght{0.03f};
+    };
+
+    LagueModel(const SPHModel::Parameters& base, const Parameters& model, const SimulationParameters& global,
+               uint32_t flags = 0);
+    ~LagueModel() override;
+
+    void Init() override;
+    void Step(hipStream_t stream) override;
+    void Clear() override;
+
+    const Parameters& GetModelParameters() const { return model_parameters; }
+    void SetModelParameters(const Parameters& p);
+    vfs_params RawParameters() const;
+
+private:
+    Parameters model_parameters;
+    SimulationParameters global;
+    uint32_t flags;
+};
+
 class SceneBase {
 public:
     virtual ~SceneBase() = default;
@@ -180,6 +209,24 @@ private:
     std::vector<FluidBlock> blocks;
     std::vector<ObjectDef> object_defs;
     float cell_size;
+    uint32_t flags;
+};
+
+// DamBreakScene (src/scenes/dam_break_scene.cpp:8-50): a 50 x 60 x 50 block of
+// the Lague model at target density 630 in a 23 x 10 x 10 box, centred in z,
+// dt = 1/120 over 3 iterations; gravity and smoothing radius from the
+// Simulation's global parameters.
+class DamBreakScene : public SceneBase {
+public:
+    explicit DamBreakScene(uint32_t flags = 0) : flags(flags) {}
+
+    void Init() override;
+    void Step(hipStream_t stream) override;
+    void Clear() override;
+    void Reset() override;
+
+private:
+    uvec3 fluid_block_size{50, 60, 50};
     uint32_t flags;
 };
 

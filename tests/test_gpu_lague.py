@@ -148,3 +148,31 @@ def test_lague_neighbour_list_overflow_bit_exact(part):
     sim.substeps(4)
     orc.substeps(4)
     _same(sim, orc)
+
+
+def test_cpp_dam_break_scene_matches_the_c_abi(tmp_path):
+    """tests/cpp/facade_lague (DamBreakScene with LagueModel in the C++ facade,
+    dam_break_scene.cpp) gives the same state as the reference-sized Lague dam
+    break through ctypes (configs.lague_dam_break), bit for bit."""
+    import subprocess
+    from pathlib import Path
+    exe = Path(__file__).resolve().parent / "cpp" / "facade_lague"
+    out = tmp_path / "lague.bin"
+    print("case facade DamBreakScene: 150,000 particles, 4 frames", flush=True)
+    r = subprocess.run([str(exe), "4", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = out.read_bytes()
+    n = int(np.frombuffer(raw[:4], np.uint32)[0])
+    f = np.frombuffer(raw[4:], np.float32)
+    x, v, rho = f[:3 * n].reshape(n, 3), f[3 * n:6 * n].reshape(n, 3), f[6 * n:7 * n]
+    w = CF.lague_dam_break()
+    assert n == w.n
+    sim = V.Lague(_params(w, flags=0))
+    sim.set_state(w.x, w.v)
+    for _ in range(4):
+        sim.step()
+    ex, ev, er, _ = sim.get_state()
+    np.testing.assert_array_equal(x, ex)
+    np.testing.assert_array_equal(v, ev)
+    np.testing.assert_array_equal(rho, er)
+    sim.close()

@@ -89,15 +89,16 @@ def test_trajectory_argument_checks(tmp_path):
 
 
 def test_cpp_facade_example_links():
-    """The C++ facade example links against libvfs.so and starts (usage path,
+    """The C++ facade examples link against libvfs.so and start (usage path,
     no device work)."""
     import subprocess
     from pathlib import Path
-    exe = Path(__file__).resolve().parent / "cpp" / "facade_scene"
-    if not exe.exists():
-        subprocess.run(["make", "-C", str(exe.parent)], check=True, capture_output=True)
-    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 2 and "usage" in r.stderr
+    for name in ("facade_scene", "facade_lague"):
+        exe = Path(__file__).resolve().parent / "cpp" / name
+        if not exe.exists():
+            subprocess.run(["make", "-C", str(exe.parent)], check=True, capture_output=True)
+        r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "usage" in r.stderr
 
 
 def test_render_buffer_argument_checks():

@@ -7,6 +7,8 @@ the reference seeds them (src/models/model.cpp:52-63).
 """
 from __future__ import annotations
 
+import ctypes
+import ctypes.util
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -190,11 +192,23 @@ def obstacle_1m(spacing: float = 0.006, map_res: int = 128, side=(128, 64, 128),
                     shapes)
 
 
+def _cbrtf(x: np.float32) -> np.float32:
+    """std::cbrt of a float as the reference's host code computes it (the C
+    library's cbrtf; numpy's float32 cbrt can differ in the last bit)."""
+    try:
+        libm = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+        libm.cbrtf.restype = ctypes.c_float
+        libm.cbrtf.argtypes = [ctypes.c_float]
+        return np.float32(libm.cbrtf(float(x)))
+    except (OSError, AttributeError):
+        return np.float32(np.cbrt(np.float64(x)))
+
+
 def compact_block(counts, density: float, pos=(0.0, 0.0, 0.0)) -> np.ndarray:
     """SpawnCompactParticlesInBox (model.cpp:36-63) for the box a scene sizes
     from `counts` (dam_break_scene.cpp:42-50): spacing cbrt(1 / density) in
     float32, x-major with z innermost, at the box position."""
-    step = np.float32(np.cbrt(np.float32(1.0) / np.float32(density)))
+    step = _cbrtf(np.float32(1.0) / np.float32(density))
     size = step * (np.asarray(counts, np.float32) + np.float32(1.0))
     n_axis = [int(np.round(size[a] / step)) - 1 for a in range(3)]
     i, j, k = np.meshgrid(*[np.arange(m, dtype=np.float32) for m in n_axis], indexing="ij")
@@ -208,7 +222,7 @@ def lague_dam_break(counts=(50, 60, 50), steps: int = 100) -> Workload:
     dt = 1/120 over 3 iterations, smooth radius 0.2, the block centred in z."""
     h, rho0 = 0.2, 630.0
     box = (23.0, 10.0, 10.0)
-    step = np.float32(np.cbrt(np.float32(1.0) / np.float32(rho0)))
+    step = _cbrtf(np.float32(1.0) / np.float32(rho0))
     size_z = step * np.float32(counts[2] + 1)
     pos = (0.0, 0.0, float(np.float32(0.0) + (np.float32(box[2]) - size_z) / np.float32(2.0)))
     x = compact_block(counts, rho0, pos)

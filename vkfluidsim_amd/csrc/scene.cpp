@@ -369,7 +369,95 @@ void WCSPHWithBoundaryModel::SetModelParameters(const Parameters& mp) {
     }
 }
 
+LagueModel::LagueModel(const SPHModel::Parameters& base, const Parameters& model, const SimulationParameters& g,
+                       uint32_t fl)
+    : model_parameters(model), global(g), flags(fl) {
+    parameters = base;
+}
+
+LagueModel::~LagueModel() { Clear(); }
+
+vfs_params LagueModel::RawParameters() const {
+    vfs_params p{};
+    for (int a = 0; a < 3; a++) {
+        p.gravity[a] = global.gravity[a];
+        p.bbox_size[a] = parameters.bounding_box.size[a];
+        p.bbox_pos[a] = parameters.bounding_box.pos[a];
+    }
+    p.smooth_radius = global.smooth_radius;
+    p.time_scale = parameters.time_scale;
+    p.iterations = parameters.iterations;
+    p.n_particles = parameters.n_particles;
+    p.fixed_dt = parameters.fixed_dt;
+    p.target_density = parameters.target_density;
+    p.flags = flags;
+    return p;
+}
+
+static vfs_lague_params to_lague(const LagueModel::Parameters& m) {
+    return vfs_lague_params{m.wall_damping_factor, m.pressure_multiplier, m.near_pressure_multiplier,
+                            m.viscosity_strenght};
+}
+
+void LagueModel::Init() {
+    Clear();
+    const vfs_params p = RawParameters();
+    const vfs_lague_params l = to_lague(model_parameters);
+    Check(vfs_lague_create(&p, &l, &sim), "vfs_lague_create");
+}
+
+void LagueModel::Step(hipStream_t stream) { Check(vfs_sim_step(sim, stream), "vfs_sim_step"); }
+
+void LagueModel::Clear() {
+    if (sim) vfs_sim_destroy(sim);
+    sim = nullptr;
+}
+
+void LagueModel::SetModelParameters(const Parameters& mp) {
+    model_parameters = mp;
+    if (sim) {
+        const vfs_lague_params l = to_lague(model_parameters);
+        Check(vfs_lague_set_params(sim, &l), "vfs_lague_set_params");
+    }
+}
+
 // ---------------------------------------------------------------- scenes
+void DamBreakScene::Init() {
+    SPHModel::Parameters base;
+    base.time_scale = 1.0f;
+    base.fixed_dt = 1.0f / 120.0f;
+    base.iterations = 3;
+    base.n_particles = (int)(fluid_block_size[0] * fluid_block_size[1] * fluid_block_size[2]);
+    base.target_density = 630.0f;
+    base.bounding_box = BoundingBox{{23.0f, 10.0f, 10.0f}, {0.0f, 0.0f, 0.0f}};
+    time_step_model = std::make_unique<LagueModel>(base, LagueModel::Parameters{},
+                                                   Simulation::Get().GetGlobalParameters(), flags);
+    time_step_model->Init();
+    Reset();
+}
+
+void DamBreakScene::Step(hipStream_t stream) { time_step_model->Step(stream); }
+
+void DamBreakScene::Clear() {
+    if (time_step_model) time_step_model->Clear();
+}
+
+// DamBreakScene::Reset (dam_break_scene.cpp:34-48): a box one particle
+// diameter larger than the block per axis, centred in z.
+void DamBreakScene::Reset() {
+    SPHModel* m = time_step_model.get();
+    const BoundingBox box = m->GetBoundingBox();
+    const float particle_volume = 1 / m->GetParameters().target_density;
+    const float particle_diam = std::cbrt(particle_volume);
+    BoundingBox block;
+    for (int a = 0; a < 3; a++) {
+        block.size[a] = particle_diam * (float)(fluid_block_size[a] + 1u);
+        block.pos[a] = box.pos[a];
+    }
+    block.pos[2] += (box.size[2] - block.size[2]) / 2.0f;
+    m->SetParticlesInBox(block);
+}
+
 GenericScene::GenericScene(const SPHModel::Parameters& b, const WCSPHWithBoundaryModel::Parameters& m,
                            const SimulationParameters& g, std::vector<FluidBlock> bl, std::vector<ObjectDef> od,
                            float cs, uint32_t fl)
