@@ -127,6 +127,58 @@ def measure_traffic(workload: str, timeout: float = 240.0) -> dict:
     return out
 
 
+# VALU issue of the neighbour kernels: one more child run under rocprofv3
+# --pmc with these SQ counters (totals over the device per dispatch)
+VALU_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU")
+SIMDS = 256 * 4          # MI355X: 256 CUs of 4 SIMD16 units; a wave64 VALU instruction issues over 4 cycles
+PEAK_CLOCK_GHZ = 2.4     # MI355X peak engine clock
+
+
+def measure_valu(workload: str, timeout: float = 240.0) -> dict:
+    """Per launch of the density and force kernels: VALU instructions per wave,
+    the fraction of lanes active in them, and the VALU instructions as cycles
+    per SIMD (4 per wave64 instruction) for bench-side timing to compare with
+    the kernel's duration.  {} when rocprofv3 is unavailable or fails."""
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not Path(prof).exists():
+        return {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        d = Path(tmp) / "valu"
+        cmd = [prof, "--pmc", *VALU_COUNTERS, "-d", str(d), "-o", "run", "--", sys.executable,
+               str(ROOT / "bench.py"), "--traffic-probe", "--workload", workload]
+        try:
+            subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout)
+        except subprocess.TimeoutExpired:
+            return {}
+        dbs = list(d.rglob("*results.db"))
+        if not dbs:
+            return {}
+        rows = sqlite3.connect(str(dbs[0])).execute(
+            "select kernel_name, counter_name, value from counters_collection").fetchall()
+    out = {}
+    for stage in ("density", "forces_integrate"):
+        prefix = STAGE_KERNELS[stage]
+        per = {}
+        for name, counter, value in rows:
+            if kernel_base(name) == prefix:
+                per.setdefault(counter, []).append(value)
+        if not all(c in per for c in VALU_COUNTERS):
+            continue
+        mean = {c: sum(v) / len(v) for c, v in per.items()}
+        out[stage] = {
+            "valu_insts_per_wave": mean["SQ_INSTS_VALU"] / mean["SQ_WAVES"],
+            "lane_utilization": mean["SQ_THREAD_CYCLES_VALU"] / (64.0 * mean["SQ_ACTIVE_INST_VALU"]),
+            "valu_cycles_per_simd": 4.0 * mean["SQ_INSTS_VALU"] / SIMDS,
+        }
+    return out
+
+
 def traffic_probe(workload: str) -> int:
     """Child of measure_traffic: one calibration copy, then two substeps."""
     import vkfluidsim_amd as V
@@ -441,6 +493,7 @@ def main() -> int:
         return traffic_probe(args.workload)
 
     traffic = {} if args.no_traffic else measure_traffic(args.workload)
+    valu = {} if args.no_traffic else measure_valu(args.workload)
 
     import vkfluidsim_amd as V
     from vkfluidsim_amd import configs as CF
@@ -515,6 +568,15 @@ def main() -> int:
                      "traffic": traffic.get(dom) if dom in traffic else None,
                      "algorithmic_bytes": sb[dom] * w.n, "bytes_per_particle": round(sb[dom], 2),
                      "launch_ms": dom_launch_ms},
+        # the neighbour kernels are bound by VALU issue, not HBM: VALU busy =
+        # VALU cycles per SIMD / (launch time x peak clock)
+        "valu_issue": {k: {"valu_insts_per_wave": round(v["valu_insts_per_wave"]),
+                           "lane_utilization": round(v["lane_utilization"], 3),
+                           "valu_busy_frac": round(v["valu_cycles_per_simd"]
+                                                   / (per[k] / max(1, launches.get(k, 1)) * 1e-3
+                                                      * PEAK_CLOCK_GHZ * 1e9), 3),
+                           "clock_ghz_assumed": PEAK_CLOCK_GHZ}
+                       for k, v in valu.items() if k in per} or None,
         "whole_step_hbm_frac": B_ALG * value / HBM_PEAK,
         "memory_class_hbm_frac": (mem_bytes / (mem_ms * 1e-3)) / HBM_PEAK if mem_ms > 0 else None,
         "traffic_per_launch": {k: round(v) for k, v in traffic.items() if k in STAGE_KERNELS} or traffic or None,
