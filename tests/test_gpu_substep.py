@@ -430,3 +430,27 @@ def test_resort_after_state_in_any_order():
     sim2.substeps(4)
     orc2.substeps(4)
     _assert_same_step(sim2, orc2)
+
+
+def test_resort_every_key_changed_at_300k():
+    """The multi-tile re-sort with nearly every key changed: 300,001 particles
+    (a ragged count) shuffled after their first substeps, so that almost every
+    key differs from the previous sorted key at its index."""
+    rng = np.random.default_rng(11)
+    n = 300_001
+    print(f"case re-sort shuffled: {n} particles", flush=True)
+    xs = rng.uniform(0.05, 0.95, size=(n, 3)).astype(np.float32)
+    vs = rng.normal(0.0, 0.5, size=(n, 3)).astype(np.float32)
+    p = V.make_params(n, 0.03, dt=1e-4, box_size=(1.0, 1.0, 1.0), flags=V.VFS_FLAG_TRACK_IDS)
+    sim = V.Sim(p, [])
+    orc = H.OracleSim(p, [])
+    for s in (sim, orc):
+        s.set_state(xs, vs)
+        s.substeps(2)
+    x, v, _, _ = sim.get_state()
+    perm = rng.permutation(n)
+    for s in (sim, orc):
+        s.set_state(x[perm], v[perm])
+        s.substeps(2)
+    _assert_same_step(sim, orc)
+    sim.close()
