@@ -179,6 +179,28 @@ def measure_valu(workload: str, timeout: float = 240.0) -> dict:
     return out
 
 
+def candidates_per_particle(x: np.ndarray, cell: float, n: int) -> float:
+    """Mean candidates that a particle's cell walk tests on the state x: the
+    sizes of the 27 buckets of its stencil, summed (a bucket is visited again
+    when two stencil cells share a key), with the substep's cell hash
+    (HashCell3D mod n) restated in numpy."""
+    c = np.floor(x / np.float32(cell)).astype(np.int64)
+
+    def key(cc):
+        u = (cc + 25) & 0xFFFFFFFF
+        lo, hi = u % 50, u // 50
+        h = lo[:, 0] + 50 * (lo[:, 1] + 50 * lo[:, 2]) + hi[:, 0] * 15823 + hi[:, 1] * 9737333 + hi[:, 2] * 440817757
+        return (h & 0xFFFFFFFF) % n
+
+    counts = np.bincount(key(c), minlength=n)
+    total = 0
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                total += int(counts[key(c + np.array([dx, dy, dz], np.int64))].sum())
+    return total / max(1, len(x))
+
+
 def traffic_probe(workload: str) -> int:
     """Child of measure_traffic: one calibration copy, then two substeps."""
     import vkfluidsim_amd as V
@@ -528,6 +550,7 @@ def main() -> int:
                  "outside_box": int(np.any((x < lo) | (x > hi), axis=1).sum()),
                  "rho_mean": float(rho.mean()) if w.n else 0.0,
                  "neighbour_list": {"mean": round(list_mean, 3), "max": list_max, "overflowed": list_over}}
+    cand = candidates_per_particle(x, w.h, w.n) if valu and w.n else None
     stages = sim.profile(args.profile_steps)
     sim.close()
 
@@ -569,8 +592,13 @@ def main() -> int:
                      "algorithmic_bytes": sb[dom] * w.n, "bytes_per_particle": round(sb[dom], 2),
                      "launch_ms": dom_launch_ms},
         # the neighbour kernels are bound by VALU issue, not HBM: VALU busy =
-        # VALU cycles per SIMD / (launch time x peak clock)
+        # VALU cycles per SIMD / (launch time x peak clock); per work item:
+        # density per candidate tested, forces per listed neighbour
         "valu_issue": {k: {"valu_insts_per_wave": round(v["valu_insts_per_wave"]),
+                           "work_per_particle": ({"candidates": round(cand, 1)} if k == "density"
+                                                 else {"listed_neighbours": round(list_mean, 1)}),
+                           "valu_insts_per_item": round(v["valu_insts_per_wave"]
+                                                        / ((cand or 1.0) if k == "density" else max(list_mean, 1.0)), 1),
                            "lane_utilization": round(v["lane_utilization"], 3),
                            "valu_busy_frac": round(v["valu_cycles_per_simd"]
                                                    / (per[k] / max(1, launches.get(k, 1)) * 1e-3
