@@ -35,13 +35,16 @@ HBM_PEAK = 8.0e12
 B_ALG = 172
 
 
-def stage_bytes(list_mean: float) -> dict:
+def stage_bytes(list_mean: float, reorder_in_sort: bool = False) -> dict:
     """Compulsory bytes per particle of each stage of this engine, with the
     neighbour list (4 bytes per entry, list_mean entries per particle) that the
-    density pass writes and the force pass reads."""
+    density pass writes and the force pass reads.  A re-sorting substep
+    reorders the state in the sort's merge kernel (reorder_in_sort): the
+    reorder's bytes less the permutation's re-read count to the sort."""
     L = 4.0 * list_mean
     return {
-        "sort": 28.0,              # keys in, sorted keys + permutation out, table fill
+        # keys in, sorted keys + permutation out, table fill
+        "sort": 28.0 + (48.0 if reorder_in_sort else 0.0),
         "cell_ranges": 8.0,        # offsets table in, cell starts out
         "reorder": 52.0,           # permutation 4, gather x v 32, write x v 32 (float4) less the w lanes
         # positions 16 in; rho 4, boundary (grad, psi) 16, list count 4, EOS terms 8 out; list out
@@ -51,9 +54,11 @@ def stage_bytes(list_mean: float) -> dict:
     }
 
 
-# kernel-name prefixes of the stages in the rocprofv3 databases
-STAGE_KERNELS = {"density": "k_density", "forces_integrate": "k_forces_integrate", "reorder": "k_reorder_state",
-                 "cell_ranges": "k_cell_start_scan", "sort": "k_radix_pass"}
+# kernel-name prefixes of the stages of a re-sorting substep (every substep
+# after the first) in the rocprofv3 databases; the merge also reorders x and v
+STAGE_KERNELS = {"density": "k_density", "forces_integrate": "k_forces_integrate",
+                 "sort_merge": "k_resort_merge", "cell_ranges": "k_resort_cell_starts",
+                 "sort_count": "k_resort_count", "sort_compact": "k_resort_compact"}
 CALIBRATION_MIB = 256
 
 
@@ -561,9 +566,9 @@ def main() -> int:
     per["sort"] = sort_ms
     dom = max(per, key=per.get)
     dom_launch_ms = per[dom] / max(1, launches.get(dom, 1)) if dom != "sort" else sort_ms
-    sb = stage_bytes(list_mean)
+    sb = stage_bytes(list_mean, reorder_in_sort="reorder" not in per)
     achieved = sb[dom] * w.n / (dom_launch_ms * 1e-3)
-    mem_class = ["sort", "cell_ranges", "reorder"]
+    mem_class = [k for k in ("sort", "cell_ranges", "reorder") if k in per]
     mem_ms = sum(per[k] for k in mem_class)
     mem_bytes = sum(sb[k] for k in mem_class) * w.n
 

@@ -121,15 +121,8 @@ struct vfs_sim {
     // otherwise a full sort (whose preparation kernel resets the offsets
     // table to n), the reorder's key starts and the reverse min-scan.
     int sort_reorder_ranges(const StepBuffers& b, hipStream_t s, StageTimer* timer) {
-        if (resort_ready) {
-            VFS_TRY(resort_keys_iota(plan, keys, sorted_keys, perm, cell_start, sort_ws, s, timer));
-            StepBuffers bo = b;
-            bo.offsets = nullptr;
-            if (timer) timer->begin(s, "reorder");
-            VFS_TRY(launch_reorder_state(bo, s));
-            if (timer) timer->end(s);
-            return 0;
-        }
+        if (resort_ready)  // the merge of the re-sort also reorders the state
+            return resort_keys_iota(plan, keys, sorted_keys, perm, cell_start, sort_ws, s, timer, &b);
         VFS_TRY(sort_keys_iota(plan, keys, sorted_keys, perm, sort_ws, s, timer, offsets, (size_t)n + 1, n));
         if (timer) timer->begin(s, "reorder");
         VFS_TRY(launch_reorder_state(b, s));  // + key starts into the offsets table

@@ -675,7 +675,11 @@ __global__ __launch_bounds__(256) void k_resort_compact(const uint32_t* __restri
 
 // Output positions of both runs: threads below the changed count place the
 // sorted changed entry of their index, every unchanged index finds how many
-// changed entries precede it by binary search.
+// changed entries precede it by binary search.  MOVE: each placed entry also
+// moves its particle's rows to the sorted place (Reorder, reorder.slang:11-19,
+// as a scatter), x and v (and ids, predictions when present) from mv's
+// current-order arrays into its sorted ones.
+template <bool MOVE>
 __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict__ keys,
                                                       const uint32_t* __restrict__ nb, uint32_t n,
                                                       const uint32_t* __restrict__ state,
@@ -683,7 +687,7 @@ __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ bis,
                                                       const uint32_t* __restrict__ cell_start,
                                                       uint32_t* __restrict__ keys_out,
-                                                      uint32_t* __restrict__ perm_out) {
+                                                      uint32_t* __restrict__ perm_out, StepBuffers mv) {
     const uint32_t g = blockIdx.x * (uint32_t)kThreads + threadIdx.x;
     if (g >= n) return;
     const uint32_t count = state[0];
@@ -695,11 +699,26 @@ __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict
         if (pos < n) {  // always, for an ascending previous key array and its bucket starts
             keys_out[pos] = kb;
             perm_out[pos] = ib;
+            if (MOVE) {
+                mv.xs[pos] = mv.x[ib];
+                mv.vs[pos] = mv.v[ib];
+                if (mv.pid) mv.pids[pos] = mv.pid[ib];
+                if (mv.ps) mv.ps[pos] = mv.pred[ib];
+            }
         }
     }
     const uint32_t e0 = nb[g];
     if (nb[g + 1u] != e0) return;  // changed: placed by its sorted entry
     const uint32_t kj = keys[g];
+    // the row of an unchanged entry is read before its place is searched
+    float4 xg, vg, pg;
+    uint32_t ig = 0;
+    if (MOVE) {
+        xg = mv.x[g];
+        vg = mv.v[g];
+        if (mv.pid) ig = mv.pid[g];
+        if (mv.ps) pg = mv.pred[g];
+    }
     uint32_t lo = 0, hi = count;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -713,6 +732,12 @@ __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict
     if (out < n) {
         keys_out[out] = kj;
         perm_out[out] = g;
+        if (MOVE) {
+            mv.xs[out] = xg;
+            mv.vs[out] = vg;
+            if (mv.pid) mv.pids[out] = ig;
+            if (mv.ps) mv.ps[out] = pg;
+        }
     }
 }
 
@@ -879,7 +904,7 @@ static int radix_sort(const SortPlan& p, int bits, int passes, const uint32_t* k
 // previous sorted keys in keys_out and their bucket starts.
 static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
                      uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer,
-                     uint32_t* fill, size_t fill_words, uint32_t fill_value) {
+                     uint32_t* fill, size_t fill_words, uint32_t fill_value, const StepBuffers* move) {
     const uint32_t n = p.n;
     if (n == 0) return 0;
     if (!cell_start) {
@@ -922,8 +947,12 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
     VFS_TRY(radix_sort(p, p.rs_bits, p.rs_passes, bkeys, bvals, bks, bis, state, workspace, s, timer, true,
                        kSmallSort));
     if (timer) timer->begin(s, "sort_merge");
-    hipLaunchKernelGGL(k_resort_merge, dim3(grid_for(n)), dim3(256), 0, s, keys_in, nb, n, state, bks, bis,
-                       cell_start, keys_out, perm_out);
+    if (move)
+        hipLaunchKernelGGL(k_resort_merge<true>, dim3(grid_for(n)), dim3(256), 0, s, keys_in, nb, n, state, bks,
+                           bis, cell_start, keys_out, perm_out, *move);
+    else
+        hipLaunchKernelGGL(k_resort_merge<false>, dim3(grid_for(n)), dim3(256), 0, s, keys_in, nb, n, state, bks,
+                           bis, cell_start, keys_out, perm_out, StepBuffers{});
     VFS_TRY(check_launch("sort_merge"));
     if (timer) timer->end(s);
     if (timer) timer->begin(s, "cell_ranges");
@@ -937,12 +966,14 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
 int sort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
                    uint32_t* workspace, hipStream_t s, StageTimer* timer, uint32_t* fill, size_t fill_words,
                    uint32_t fill_value) {
-    return sort_impl(p, keys_in, keys_out, perm_out, nullptr, workspace, s, timer, fill, fill_words, fill_value);
+    return sort_impl(p, keys_in, keys_out, perm_out, nullptr, workspace, s, timer, fill, fill_words, fill_value,
+                     nullptr);
 }
 
 int resort_keys_iota(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
-                     uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer) {
-    return sort_impl(p, keys_in, keys_out, perm_out, cell_start, workspace, s, timer, nullptr, 0, 0);
+                     uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer,
+                     const StepBuffers* move) {
+    return sort_impl(p, keys_in, keys_out, perm_out, cell_start, workspace, s, timer, nullptr, 0, 0, move);
 }
 
 int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,

@@ -57,9 +57,14 @@ int sort_keys_iota(const SortPlan& plan, const uint32_t* keys_in, uint32_t* keys
 // equal to the previous key at their index keep their relative order, the
 // changed ones are radix-sorted on their own (device-side count) and merged;
 // cell_start is then updated in place to the bucket starts of the new keys
-// (plan.n + 1 entries; no offsets table is written).
+// (plan.n + 1 entries; no offsets table is written).  With move, the merge
+// also reorders the particle state: move->xs[i] = move->x[perm_out[i]], the
+// same for v, pid and pred when present (each row written by the thread that
+// places its key, so no separate gather over the permutation).
+struct StepBuffers;
 int resort_keys_iota(const SortPlan& plan, const uint32_t* keys_in, uint32_t* keys_out, uint32_t* perm_out,
-                     uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer);
+                     uint32_t* cell_start, uint32_t* workspace, hipStream_t s, StageTimer* timer,
+                     const StepBuffers* move = nullptr);
 
 // keys[i] = key_base + HashCell3D(floor(pos_i / cell)) % table_size
 int launch_hash_keys(const float* pos, uint32_t stride, uint32_t n, float cell, uint32_t table_size,
