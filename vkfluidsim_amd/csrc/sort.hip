@@ -37,6 +37,7 @@ __device__ __forceinline__ void st_release_agent(uint32_t* p, uint32_t v) {
 }
 
 // Exclusive scan of one value per thread over a 256-thread block.
+template <int WAVES = kWaves>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t& total) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     uint32_t x = v;
@@ -49,7 +50,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     __syncthreads();
     uint32_t wave_off = 0, tot = 0;
 #pragma unroll
-    for (int i = 0; i < kWaves; i++) {
+    for (int i = 0; i < WAVES; i++) {
         const uint32_t t = s_tmp[i];
         if (i < w) wave_off += t;
         tot += t;
@@ -250,26 +251,31 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
 // (count read on the device) in one block's LDS: each pass ranks the keys as
 // k_radix_pass ranks a tile (wave ballots; wave-major, item-major, lane order =
 // input order) and scatters them within LDS.  A larger count leaves at once
-// (the multi-tile passes sort it).
-constexpr int kSmallItems = 16;
-constexpr uint32_t kSmallSort = (uint32_t)kThreads * kSmallItems;
-template <int BITS>
-__global__ __launch_bounds__(256) void k_resort_small(const uint32_t* __restrict__ keys_in,
-                                                      const uint32_t* __restrict__ vals_in,
-                                                      uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
-                                                      const uint32_t* __restrict__ n_dev, int passes) {
+// (the multi-tile passes sort it).  THREADS = 1024: sixteen waves of four
+// items each keep the CU's four SIMDs busy on the serial per-item ranking
+// (96 KB of LDS for 10-bit digits).
+constexpr uint32_t kSmallSort = 4096;
+constexpr int kSmallThreads = 1024;
+template <int BITS, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_resort_small(const uint32_t* __restrict__ keys_in,
+                                                          const uint32_t* __restrict__ vals_in,
+                                                          uint32_t* __restrict__ keys_out,
+                                                          uint32_t* __restrict__ vals_out,
+                                                          const uint32_t* __restrict__ n_dev, int passes) {
     constexpr int RADIX = 1 << BITS;
-    constexpr int DPT = RADIX / kThreads;
-    constexpr int ITEMS = kSmallItems;
-    __shared__ uint32_t s_wave[kWaves][RADIX];
+    constexpr int WAVES = THREADS / 64;
+    constexpr int DPT = RADIX / THREADS;
+    constexpr int ITEMS = (int)kSmallSort / THREADS;
+    static_assert(DPT >= 1 && ITEMS >= 1 && ITEMS * THREADS == (int)kSmallSort, "block shape");
+    __shared__ uint32_t s_wave[WAVES][RADIX];
     __shared__ uint32_t s_local[RADIX];
     __shared__ uint32_t s_keys[kSmallSort];
     __shared__ uint32_t s_vals[kSmallSort];
-    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t s_tmp[WAVES];
     const uint32_t n = *n_dev;
     if (n == 0 || n > kSmallSort) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (uint32_t i = (uint32_t)tid; i < n; i += kThreads) {
+    for (uint32_t i = (uint32_t)tid; i < n; i += THREADS) {
         s_keys[i] = keys_in[i];
         s_vals[i] = vals_in[i];
     }
@@ -277,7 +283,7 @@ __global__ __launch_bounds__(256) void k_resort_small(const uint32_t* __restrict
     const uint32_t wave_base = (uint32_t)w * 64u * ITEMS;
     for (int pass = 0; pass < passes; pass++) {
         const int shift = pass * BITS;
-        for (int i = tid; i < kWaves * RADIX; i += kThreads) (&s_wave[0][0])[i] = 0;
+        for (int i = tid; i < WAVES * RADIX; i += THREADS) (&s_wave[0][0])[i] = 0;
         __syncthreads();  // the clear (and, before the first pass, the loads) are done
         uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
 #pragma unroll
@@ -315,7 +321,7 @@ __global__ __launch_bounds__(256) void k_resort_small(const uint32_t* __restrict
             const int d = tid * DPT + dd;
             uint32_t c = 0;
 #pragma unroll
-            for (int ww = 0; ww < kWaves; ww++) {
+            for (int ww = 0; ww < WAVES; ww++) {
                 const uint32_t x = s_wave[ww][d];
                 s_wave[ww][d] = c;
                 c += x;
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(256) void k_resort_small(const uint32_t* __restrict
             tsum += c;
         }
         uint32_t tot;
-        uint32_t lrun = block_exclusive_scan(tsum, s_tmp, tot);
+        uint32_t lrun = block_exclusive_scan<WAVES>(tsum, s_tmp, tot);
 #pragma unroll
         for (int dd = 0; dd < DPT; dd++) {
             s_local[tid * DPT + dd] = lrun;
@@ -343,7 +349,7 @@ __global__ __launch_bounds__(256) void k_resort_small(const uint32_t* __restrict
         }
         __syncthreads();  // the scatter is done before the next pass clears s_wave and reads s_keys
     }
-    for (uint32_t i = (uint32_t)tid; i < n; i += kThreads) {
+    for (uint32_t i = (uint32_t)tid; i < n; i += THREADS) {
         keys_out[i] = s_keys[i];
         vals_out[i] = s_vals[i];
     }
@@ -941,7 +947,8 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
     // a few changed keys are sorted by one block in LDS, more by the radix passes
     static_assert(kMaxPasses * 1024 >= 3 * 1024, "histograms of the 10-bit re-sort digits fit");
     if (timer) timer->begin(s, "sort_small");
-    hipLaunchKernelGGL(k_resort_small<10>, dim3(1), dim3(256), 0, s, bkeys, bvals, bks, bis, state, p.rs_passes);
+    hipLaunchKernelGGL((k_resort_small<10, kSmallThreads>), dim3(1), dim3(kSmallThreads), 0, s, bkeys, bvals, bks,
+                       bis, state, p.rs_passes);
     VFS_TRY(check_launch("sort_small"));
     if (timer) timer->end(s);
     VFS_TRY(radix_sort(p, p.rs_bits, p.rs_passes, bkeys, bvals, bks, bis, state, workspace, s, timer, true,
