@@ -185,8 +185,10 @@ int vfs_sim_set_tuning(vfs_sim *sim, const vfs_tuning *tuning);
 int vfs_sim_autotune(vfs_sim *sim, uint32_t substeps_per_trial, vfs_tuning *best, float *best_ms_per_substep);
 /* Runs n substeps eagerly (advancing the state) with a HIP event pair around
  * every kernel on the simulation stream.  For each stage writes the mean time
- * per substep in ms (summed over its launches), the launches per substep and
- * the stage name (at most max_stages entries; any output array may be NULL). */
+ * per substep in ms (summed over its launches), the launches per substep
+ * (total launches / n, rounded down: a stage of the first substep's full sort
+ * alone, such as "reorder" or "sort_histogram", reports 0 when n > 1) and the
+ * stage name (at most max_stages entries; any output array may be NULL). */
 int vfs_sim_profile_stages(vfs_sim *sim, uint32_t n, float *ms, uint32_t *launches,
                            const char **names, uint32_t max_stages, uint32_t *n_stages);
 /* Neighbour lists of the last substep: mean and largest entries per particle
