@@ -121,7 +121,8 @@ __device__ __forceinline__ bool stage_row(float4* __restrict__ st, const RowRang
 
 // Visits candidates jb..je-1 in order with kUnroll loads in flight per lane:
 // whole groups first (every candidate valid), then one masked tail group whose
-// extra loads read neighbouring particles (or the padding past the array).
+// extra loads read neighbouring particles (or the padding past the array) and
+// whose validity enters the count as a bit, without a branch per candidate.
 // hit(j, q, valid) appends j to the queue when valid and in range.
 template <class Load, class Hit, class Flush>
 __device__ __forceinline__ void walk_range(uint32_t jb, uint32_t je, Load&& load, const int& cnt, Hit&& hit,
@@ -205,7 +206,7 @@ __device__ __forceinline__ uint32_t build_list(ListShared<BLOCK, STAGED>& sh, co
         const float ox = q.x - pos[0], oy = q.y - pos[1], oz = q.z - pos[2];
         const float r2 = dot3(ox, oy, oz, ox, oy, oz);
         s_j[cnt][tid] = j;
-        cnt += (valid && !(r2 > h2)) ? 1 : 0;
+        cnt += (int)((uint32_t)valid & (uint32_t)!(r2 > h2));  // no branch on valid
     };
     // complete chunks into the list in order; the remainder's three slots move
     // to the front whole (slots past the remainder are overwritten before use)
