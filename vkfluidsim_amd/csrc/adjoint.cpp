@@ -143,7 +143,7 @@ int vfs_adjoint_create(vfs_sim* sim, uint32_t n_substeps, uint32_t checkpoint_ev
         r |= dev_alloc(&s.vs, n);
         r |= dev_alloc(&s.bnd, n);
         r |= dev_alloc(&s.rho, n);
-        r |= dev_alloc(&s.nbr, (size_t)n * kNeighbourCap);
+        r |= dev_alloc(&s.nbr, (size_t)n * kNeighbourWords);
         r |= dev_alloc(&s.cnt, n);
     }
     r |= dev_alloc(&a->gx_a, n);

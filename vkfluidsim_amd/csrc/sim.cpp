@@ -264,7 +264,7 @@ int create_impl(const vfs_params* params, uint32_t n_scenes, const vfs_boundary_
     r |= dev_alloc(&s->sorted_keys, n);
     r |= dev_alloc(&s->perm, n);
     r |= dev_alloc(&s->offsets, (size_t)n + 1);
-    r |= dev_alloc(&s->nbr, (size_t)n * kNeighbourCap);
+    r |= dev_alloc(&s->nbr, (size_t)n * kNeighbourWords);
     r |= dev_alloc(&s->nbr_count, n);
     r |= dev_alloc(&s->cell_start, (size_t)n + 1);
     if (s->flags & VFS_FLAG_TRACK_IDS) {

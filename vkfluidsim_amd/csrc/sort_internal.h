@@ -100,7 +100,7 @@ struct StepBuffers {
     float* rho;
     float4* bnd;         // (grad psi xyz, psi)
     float4* accel;       // nullable
-    uint32_t* nbr;       // nbr_cap * n: neighbour lists of the substep in chunks of four,
+    uint32_t* nbr;       // (nbr_cap + 4) * n: neighbour lists of the substep in chunks of four,
                          // chunk-major (uint4 chunk g of particle i at g * n + i)
     uint32_t* nbr_count; // n: list lengths (> nbr_cap: overflowed)
     // Lague model only (null otherwise): predicted positions in the current

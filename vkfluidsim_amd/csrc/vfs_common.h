@@ -128,6 +128,10 @@ struct SceneTable {
 // have ~30 neighbours).
 constexpr uint32_t kNeighbourCap = VFS_NEIGHBOUR_CAPACITY;
 static_assert(kNeighbourCap % 4 == 0, "lists are stored in chunks of four");
+// uint32 words of list storage per particle: kNeighbourCap / 4 chunk rows, then
+// one more row that takes the chunk stores of a list past the capacity (never
+// read), so that the stores need no branch
+constexpr uint32_t kNeighbourWords = kNeighbourCap + 4u;
 
 // Boundary object as the kernels see it (column-major matrices).
 struct DevObject {

@@ -214,7 +214,7 @@ __device__ __forceinline__ uint32_t build_list(ListShared<BLOCK, STAGED>& sh, co
         const int full = cnt >> 2;
         for (int g = 0; g < full; g++) {
             const uint4 v = make_uint4(s_j[4 * g][tid], s_j[4 * g + 1][tid], s_j[4 * g + 2][tid], s_j[4 * g + 3][tid]);
-            if (chunks < (c.nbr_cap >> 2)) list[(size_t)chunks * n + i] = v;
+            list[(size_t)min(chunks, c.nbr_cap >> 2) * n + i] = v;  // past the capacity: the spare row
             chunks++;
         }
         const int rem = cnt & 3;
