@@ -285,6 +285,16 @@ int vfs_spatial_hash_destroy(vfs_spatial_hash *h);
 int vfs_sort_workspace_size(uint32_t n, uint32_t max_value, size_t *bytes);
 int vfs_sort_pairs(uint32_t *items_dev, uint32_t *keys_dev, uint32_t n, uint32_t max_value,
                    void *workspace, size_t workspace_bytes, void *stream);
+/* vfs_sort_pairs without a workspace argument and without any allocation: the
+ * workspace is a fixed device array of the library (26 MiB per GPU, held until
+ * the process ends), so n is limited to 2^20 keys (VFS_ERR_INVALID beyond).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under stream capture, serving loops, harnesses that
+ * forbid hipMalloc).  All calls on one GPU share that array: calls that
+ * overlap on one GPU race, so the caller serialises them. */
+#define VFS_SORT_NOALLOC_MAX_KEYS (1u << 20)
+int vfs_sort_pairs_noalloc(uint32_t *items_dev, uint32_t *keys_dev, uint32_t n, uint32_t max_value,
+                           void *stream);
 /* GPUScan::Run (src/compute/sort.cpp:57-86): exclusive prefix sum in place. */
 int vfs_scan_workspace_size(uint32_t n, size_t *bytes);
 int vfs_scan_exclusive(uint32_t *elements_dev, uint32_t n, void *workspace, size_t workspace_bytes,

@@ -189,6 +189,7 @@ SIGNATURES = {
     "vfs_spatial_hash_destroy": (_i32, [_vp]),
     "vfs_sort_workspace_size": (_i32, [_u32, _u32, C.POINTER(_sz)]),
     "vfs_sort_pairs": (_i32, [_vp, _vp, _u32, _u32, _vp, _sz, _vp]),
+    "vfs_sort_pairs_noalloc": (_i32, [_vp, _vp, _u32, _u32, _vp]),
     "vfs_scan_workspace_size": (_i32, [_u32, C.POINTER(_sz)]),
     "vfs_scan_exclusive": (_i32, [_vp, _u32, _vp, _sz, _vp]),
     "vfs_spatial_offsets": (_i32, [_vp, _vp, _u32, _i32, _vp]),

@@ -120,7 +120,7 @@ int vfs_spatial_hash_destroy(vfs_spatial_hash* h) {
 int vfs_sort_workspace_size(uint32_t n, uint32_t max_value, size_t* bytes) {
     VFS_CHECK(bytes, VFS_ERR_INVALID, "bytes is null");
     VFS_CHECK(n < (1u << 30), VFS_ERR_INVALID, "n must be < 2^30");
-    const SortPlan p = make_sort_plan(n, max_value, 8, 16);
+    const SortPlan p = make_sort_plan(n, max_value, 8, 16, false);
     *bytes = (p.workspace_words + 2 * (size_t)n) * 4;
     return 0;
 }
@@ -133,7 +133,7 @@ int vfs_sort_pairs(uint32_t* items_dev, uint32_t* keys_dev, uint32_t n, uint32_t
     VFS_CHECK(n < (1u << 30), VFS_ERR_INVALID, "n must be < 2^30");
     if (n == 0) return 0;
     VFS_CHECK(items_dev && keys_dev && workspace, VFS_ERR_INVALID, "null argument");
-    const SortPlan p = make_sort_plan(n, max_value, 8, 16);
+    const SortPlan p = make_sort_plan(n, max_value, 8, 16, false);
     const size_t need = (p.workspace_words + 2 * (size_t)n) * 4;
     VFS_CHECK(workspace_bytes >= need, VFS_ERR_INVALID,
               "workspace too small: need " + std::to_string(need) + " bytes");

@@ -786,7 +786,7 @@ int bit_width(uint32_t v) {
 
 // ------------------------------------------------------------ host launchers
 
-SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int items) {
+SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int items, bool resort) {
     SortPlan p{};
     p.n = n;
     p.bits = bits_per_pass;
@@ -807,14 +807,16 @@ SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int i
     p.off_tmp[1] = take(n);
     p.off_tmp[2] = take(n);
     p.off_tmp[3] = take(n);
-    p.off_rs_nb = take((size_t)n + 1);
-    p.off_rs_bkeys = take(n);
-    p.off_rs_bvals = take(n);
-    p.off_rs_bks = take(n);
-    p.off_rs_bis = take(n);
+    // a plan without re-sort gives its buffers no words (they are never used)
+    const size_t rs_n = resort ? (size_t)n : 0;
+    p.off_rs_nb = take(resort ? rs_n + 1 : 0);
+    p.off_rs_bkeys = take(rs_n);
+    p.off_rs_bvals = take(rs_n);
+    p.off_rs_bks = take(rs_n);
+    p.off_rs_bis = take(rs_n);
     p.rs_bits = 10;
     p.rs_passes = kb == 0 ? 1 : (kb + p.rs_bits - 1) / p.rs_bits;
-    const size_t rs_radix = (size_t)1 << p.rs_bits;
+    const size_t rs_radix = resort ? (size_t)1 << p.rs_bits : 0;
     p.off_hist = take(kMaxPasses * std::max(radix, rs_radix));
     p.off_counters = take(kMaxPasses);
     p.off_status = take(std::max((size_t)p.passes * radix, (size_t)p.rs_passes * rs_radix) * p.num_tiles);
@@ -823,7 +825,7 @@ SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int i
     p.off_cs_counter = take(1);
     p.off_cs_status = take(p.cs_tiles);
     const uint32_t rs_tile = (uint32_t)kThreads * kResortItems;
-    p.rs_tiles = n == 0 ? 0 : (n + rs_tile - 1) / rs_tile;
+    p.rs_tiles = (n == 0 || !resort) ? 0 : (n + rs_tile - 1) / rs_tile;
     p.off_rs_tile_counts = take(p.rs_tiles);
     p.off_rs_state = take(2);
     p.zero_begin = p.off_hist;

@@ -42,7 +42,9 @@ struct SortPlan {
 };
 
 constexpr int kResortItems = 16;  // keys per thread of the re-sort's count and compaction
-SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int items);
+// resort = false: a plan for full sorts only (sort_keys_iota), without the
+// re-sort's buffers.
+SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int items, bool resort = true);
 int valid_sort_config(int bits, int items);
 
 // Stable sort of (keys_in, iota): keys_out = sorted keys, perm_out[i] = input
