@@ -114,6 +114,11 @@ public:
     const Parameters& GetModelParameters() const { return model_parameters; }
     void SetModelParameters(const Parameters& p);
     vfs_params RawParameters() const;
+    // The objects as the simulation received them (GenericScene::boundary_objects
+    // with their BoundaryObjectInfo, generic_scene.cpp:125-145).
+    const std::vector<BoundaryObject>& GetBoundaryObjects() const { return objects; }
+    // Object k as a C ABI record whose grid pointers refer to this model's host grids.
+    vfs_boundary_object BoundaryObjectRecord(size_t k) const;
 
 private:
     Parameters model_parameters;

@@ -475,6 +475,13 @@ int vfs_scene_load(const char *json_path, const char *resource_root, vfs_scene *
 int vfs_scene_sim(vfs_scene *scene, vfs_sim **sim);
 int vfs_scene_reset(vfs_scene *scene);
 int vfs_scene_step(vfs_scene *scene, void *stream);
+/* The scene's boundary objects as its simulation received them (the
+ * BoundaryObjectInfo records of GenericScene::CreateBoundaryObjectBuffer,
+ * generic_scene.cpp:125-145: world-to-object matrix inverse(T R S), rotation,
+ * grid box and resolution; grid pointers refer to the scene's host grids,
+ * valid until vfs_scene_destroy).  Writes min(capacity, count) records to
+ * out (may be NULL) and the object count to count. */
+int vfs_scene_boundary_objects(vfs_scene *scene, vfs_boundary_object *out, uint32_t capacity, uint32_t *count);
 int vfs_scene_destroy(vfs_scene *scene);
 
 #ifdef __cplusplus

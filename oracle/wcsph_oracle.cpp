@@ -1448,6 +1448,9 @@ struct MeshO {
         }
     }
 
+    // Exactly equal distances resolve to the lowest triangle index (the first
+    // found in index order); the reference's choice depends on its BVH
+    // traversal order.
     double signed_distance(const float* q) const {
         const double p[3] = {q[0], q[1], q[2]};
         double best = (double)std::numeric_limits<float>::max();
@@ -1593,6 +1596,23 @@ void* orc_sim_create(const ParamsC* p, const ObjectC* objs, uint32_t n_obj, int 
 }
 void orc_sim_destroy(void* h) { delete (OrcSim*)h; }
 
+// GradientVector (slang:68-85) of boundary object k's SDF grid at the
+// object-space position x, in the simulation's working type.
+int orc_sim_gradient_vector(void* h, uint32_t k, const double* x, double* out) {
+    auto* s = (OrcSim*)h;
+    if (s->fp64) {
+        if (k >= s->s64.m.objs.size()) return -1;
+        gradient_vector<double>(s->s64.m.objs[k], s->s64.m.h, x, out);
+        return 0;
+    }
+    if (k >= s->s32.m.objs.size()) return -1;
+    const float xf[3] = {(float)x[0], (float)x[1], (float)x[2]};
+    float g[3];
+    gradient_vector<float>(s->s32.m.objs[k], s->s32.m.h, xf, g);
+    for (int a = 0; a < 3; a++) out[a] = g[a];
+    return 0;
+}
+
 void orc_sim_set_threads(void* h, int threads) {
     auto* s = (OrcSim*)h;
     s->s32.threads = s->s64.threads = threads < 1 ? 1 : threads;
@@ -1692,10 +1712,11 @@ int orc_adjoint_run(void* h, uint32_t k, const orc::GridC* grid, const double* t
 
 // The FP64 adjoint of the FP32 trajectory: the forward runs on the FP32
 // handle h32 (the trajectory the GPU reproduces bit for bit), its tape is
-// widened to double, and the loss gradient and the adjoint use the model of
-// the FP64 handle h64 (same parameters and objects; its state is unused).
-// Separates the adjoint's arithmetic from the divergence of FP32 and FP64
-// trajectories.
+// widened to double, the loss and its gradient are evaluated as the GPU
+// evaluates them (FP32 kernel values summed in double), and the reverse
+// substeps use the model of the FP64 handle h64 (same parameters and
+// objects; its state is unused).  Separates the reverse substeps' arithmetic
+// from the divergence of FP32 and FP64 trajectories.
 int orc_adjoint_run_f32_trajectory(void* h32, void* h64, uint32_t k, const orc::GridC* grid, const double* target,
                                    double* loss, double* grad_x, double* grad_v, double* grad_nu) {
     auto* o32 = (OrcSim*)h32;
@@ -1721,10 +1742,31 @@ int orc_adjoint_run_f32_trajectory(void* h32, void* h64, uint32_t k, const orc::
         tape[s].nbr.swap(tape32[s].nbr);
     }
     tape32.clear();
-    const std::vector<double> x = widen(sim.x);
-    std::vector<double> rg;
-    orc::grid_density<double>(m, x, n, *grid, rg);
-    const size_t G = rg.size();
+    // The loss and its gradient with the kernel evaluated as the GPU does: float
+    // positions, grid points, distances and kernel values (the FP32 model),
+    // summed in double.  The comparison then isolates the reverse substeps.
+    const orc::Model<float>& mf = sim.m;
+    const size_t G = (size_t)grid->dims[0] * grid->dims[1] * grid->dims[2];
+    std::vector<double> rg(G, 0.0);
+    auto for_pairs = [&](auto&& f) {
+        for (uint32_t j = 0; j < n; j++) {
+            const float* xj = &sim.x[3 * (size_t)j];
+            int lo[3], hi[3];
+            orc::grid_range<float>(*grid, xj, mf.h, lo, hi);
+            for (int iz = lo[2]; iz <= hi[2]; iz++)
+                for (int iy = lo[1]; iy <= hi[1]; iy++)
+                    for (int ix = lo[0]; ix <= hi[0]; ix++) {
+                        float gp[3], d[3];
+                        orc::grid_point<float>(*grid, ix, iy, iz, gp);
+                        for (int a = 0; a < 3; a++) d[a] = xj[a] - gp[a];
+                        const float r2 = orc::dot3<float>(d, d);
+                        if (r2 > mf.h2) continue;
+                        const size_t idx = (size_t)ix + grid->dims[0] * ((size_t)iy + (size_t)grid->dims[1] * iz);
+                        f(j, idx, d, std::sqrt(r2));
+                    }
+        }
+    };
+    for_pairs([&](uint32_t, size_t idx, const float*, float r) { rg[idx] += (double)(mf.mass * mf.W(r)); });
     double L = 0.0;
     for (size_t g = 0; g < G; g++) {
         const double d = rg[g] - (target ? target[g] : 0.0);
@@ -1732,25 +1774,14 @@ int orc_adjoint_run_f32_trajectory(void* h32, void* h64, uint32_t k, const orc::
     }
     if (loss) *loss = L / (double)G;
     std::vector<double> gx(3 * (size_t)n, 0.0), gv(3 * (size_t)n, 0.0);
-    for (uint32_t j = 0; j < n; j++) {
-        int lo[3], hi[3];
-        orc::grid_range<double>(*grid, &x[3 * j], m.h, lo, hi);
-        for (int iz = lo[2]; iz <= hi[2]; iz++)
-            for (int iy = lo[1]; iy <= hi[1]; iy++)
-                for (int ix = lo[0]; ix <= hi[0]; ix++) {
-                    double gp[3], d[3];
-                    orc::grid_point<double>(*grid, ix, iy, iz, gp);
-                    for (int a = 0; a < 3; a++) d[a] = x[3 * j + a] - gp[a];
-                    const double r2 = orc::dot3<double>(d, d);
-                    if (r2 > m.h2) continue;
-                    const double r = std::sqrt(r2);
-                    if (!(r > 1e-9)) continue;
-                    const size_t idx = (size_t)ix + grid->dims[0] * ((size_t)iy + (size_t)grid->dims[1] * iz);
-                    const double cg = 2.0 * (rg[idx] - (target ? target[idx] : 0.0)) / (double)G;
-                    const double s2 = cg * m.mass * (m.h * m.dW(r)) / r;
-                    for (int a = 0; a < 3; a++) gx[3 * j + a] += s2 * d[a];
-                }
-    }
+    for_pairs([&](uint32_t j, size_t idx, const float* d, float r) {
+        if (!(r > 1e-9f)) return;
+        const double cg = 2.0 * (rg[idx] - (target ? target[idx] : 0.0)) / (double)G;
+        const double s2 = cg * (double)(mf.mass * (mf.h * mf.dW(r)) / r);
+        for (int a = 0; a < 3; a++) gx[3 * (size_t)j + a] += s2 * (double)d[a];
+    });
+    // the GPU hands the seed to the reverse substeps as float
+    for (double& g : gx) g = (double)(float)g;
     double nu_bar = 0.0;
     for (int s = (int)k - 1; s >= 0; s--) orc::adjoint_substep<double>(m, tape[s], n, gx, gv, nu_bar);
     for (size_t i = 0; i < 3 * (size_t)n; i++) {

@@ -34,6 +34,7 @@ def _load_oracle() -> C.CDLL:
         "orc_sample_grid_f64": (C.c_double, [fp, up, dp]),
         "orc_sim_create": (vp, [vp, vp, u32, C.c_int]),
         "orc_sim_destroy": (None, [vp]),
+        "orc_sim_gradient_vector": (C.c_int, [vp, u32, dp, dp]),
         "orc_sim_set_threads": (None, [vp, C.c_int]),
         "orc_sim_set_state": (None, [vp, fp, fp, u32, u32]),
         "orc_sim_set_state_f64": (None, [vp, dp, dp, u32, u32]),
@@ -135,6 +136,12 @@ class OracleSim:
         o = np.empty(n, np.uint32)
         ORC.orc_sim_get_hash(self.h, up(k), up(p), up(o))
         return k, p, o
+
+    def gradient_vector(self, k, x_local):
+        x = np.ascontiguousarray(x_local, np.float64)
+        out = np.empty(3)
+        assert ORC.orc_sim_gradient_vector(self.h, int(k), dp(x), dp(out)) == 0
+        return out
 
     def get_boundary(self):
         n = self.n

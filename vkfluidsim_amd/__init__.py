@@ -375,6 +375,17 @@ class Sim:
         call("vfs_sim_device_buffers", self.handle, C.byref(b))
         return b
 
+    def get_accelerations(self) -> np.ndarray:
+        """Accelerations of the last substep (n x 3; needs VFS_FLAG_STORE_ACCEL)."""
+        b = self.device_buffers()
+        if not b.accelerations:
+            raise VfsError(-3, "get_accelerations", "the simulation was created without VFS_FLAG_STORE_ACCEL")
+        call("vfs_sim_synchronize", self.handle)
+        a = np.empty(4 * self.n, np.float32)
+        if self.n:
+            call("vfs_memcpy_d2h", a.ctypes.data, b.accelerations, a.nbytes)
+        return a.reshape(self.n, 4)[:, :3].copy()
+
     def profile(self, k: int):
         """[(stage, ms per substep, launches per substep)] over k eager substeps."""
         m = 16
@@ -705,6 +716,26 @@ class Scene:
         pid = np.empty(n, np.uint32) if ids else None
         call("vfs_sim_get_state", self.sim_handle, fptr(x), fptr(v), fptr(rho), uptr(pid) if ids else None)
         return x.reshape(n, 3), v.reshape(n, 3), rho, pid
+
+    def boundary_objects(self) -> list:
+        """The scene's boundary objects as BoundaryGrid copies (grids, grid box,
+        world-to-object matrix and rotation, column-major float32)."""
+        cnt = C.c_uint32(0)
+        call("vfs_scene_boundary_objects", self.handle, None, 0, C.byref(cnt))
+        if cnt.value == 0:
+            return []
+        arr = (BoundaryObject * cnt.value)()
+        call("vfs_scene_boundary_objects", self.handle, arr, cnt.value, C.byref(cnt))
+        out = []
+        for o in arr:
+            res = tuple(int(r) for r in o.resolution)
+            m = res[0] * res[1] * res[2]
+            sdf = np.ctypeslib.as_array(o.sdf_grid, (m,)).copy()
+            vm = np.ctypeslib.as_array(o.volume_map_grid, (m,)).copy()
+            out.append(BoundaryGrid(sdf, vm, res, tuple(o.box_size), tuple(o.box_pos),
+                                    transform=np.array(o.transform, np.float32),
+                                    rotation=np.array(o.rotation, np.float32)))
+        return out
 
     def close(self) -> None:
         if getattr(self, "handle", None):

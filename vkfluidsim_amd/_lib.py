@@ -222,6 +222,7 @@ SIGNATURES = {
     "vfs_scene_sim": (_i32, [_vp, C.POINTER(_vp)]),
     "vfs_scene_reset": (_i32, [_vp]),
     "vfs_scene_step": (_i32, [_vp, _vp]),
+    "vfs_scene_boundary_objects": (_i32, [_vp, C.POINTER(BoundaryObject), _u32, _up]),
     "vfs_scene_destroy": (_i32, [_vp]),
 }
 

@@ -309,6 +309,8 @@ int upload(vfs_mesh* m, MeshView& view) {
                 const float* p = &m->pos[3 * (size_t)m->tris[3 * t + c]];
                 tri[3 * k + c] = make_float4(p[0], p[1], p[2], 0.0f);
             }
+            // the mesh index rides in the first corner's w (the search's tie-break)
+            std::memcpy(&tri[3 * k].w, &t, sizeof(t));
             pn[k] = m->normals[t];
         }
         if (hipMalloc(&m->d_nodes, m->nodes.size() * sizeof(MeshNode)) != hipSuccess ||

@@ -41,7 +41,7 @@ struct TriNormals {
 // the true one, 1-Lipschitz in the query point.
 struct MeshView {
     const MeshNode* nodes;
-    const float4* tri;  // 3 per triangle: v0, v1, v2 (xyz)
+    const float4* tri;  // 3 per triangle: v0, v1, v2 (xyz); v0.w holds the triangle's mesh index (bits)
     const TriNormals* pn;
     float centre[3];
     double diag2;

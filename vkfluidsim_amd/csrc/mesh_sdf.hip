@@ -173,20 +173,26 @@ struct Closest {
     double d2;
     float p[3];
     int entity;
-    uint32_t tri;
+    uint32_t tri;  // BVH position
+    uint32_t id;   // index of the triangle in the mesh
 };
 
-// Distance from q to triangle k (BVH order); keeps it in `best` if closer.
+// Distance from q to triangle k (BVH order); keeps it in `best` if closer, or
+// as close and earlier in the mesh (the first corner's w holds the triangle's
+// mesh index), so that exactly equal distances resolve to the lowest index
+// whatever order the search visits them in.
 __device__ __forceinline__ void try_triangle(const MeshView& m, const double* q, uint32_t k, Closest& best) {
     const float4 a = m.tri[3 * (size_t)k], b = m.tri[3 * (size_t)k + 1], cc = m.tri[3 * (size_t)k + 2];
     const double v0[3] = {a.x, a.y, a.z};
     const double e0[3] = {(double)b.x - v0[0], (double)b.y - v0[1], (double)b.z - v0[2]};
     const double e1[3] = {(double)cc.x - v0[0], (double)cc.y - v0[1], (double)cc.z - v0[2]};
     const TriDistance td = tri_distance(q, v0, e0, e1);
-    if (td.d2 < best.d2) {
+    const uint32_t id = __float_as_uint(a.w);
+    if (td.d2 < best.d2 || (td.d2 == best.d2 && id < best.id)) {
         best.d2 = td.d2;
         best.entity = td.entity;
         best.tri = k;
+        best.id = id;
         // closest point v0 + s e0 + t e1, stored as float (ClosestPointQueryResult)
         best.p[0] = (float)(v0[0] + td.s * e0[0] + td.t * e1[0]);
         best.p[1] = (float)(v0[1] + td.s * e0[1] + td.t * e1[1]);
@@ -199,7 +205,7 @@ constexpr uint32_t kNoHint = 0xffffffffu;
 // Minimum over all triangles of the computed squared distance.  `hint` (a
 // triangle near the query, e.g. the previous query's closest) is tried first
 // so that the search starts with a tight bound; among exactly equal
-// distances the first found is kept.  A subtree is skipped only when its box
+// distances the lowest mesh index is kept.  A subtree is skipped only when its box
 // is farther than the best distance plus a slack far above the rounding of
 // the triangle distances, so no triangle whose computed distance beats the
 // best is ever skipped.
@@ -212,6 +218,7 @@ __device__ Closest mesh_closest(const MeshView& m, float qx, float qy, float qz,
     best.p[0] = best.p[1] = best.p[2] = 0.0f;
     best.entity = kF;
     best.tri = 0;
+    best.id = 0xffffffffu;
     if (hint != kNoHint) try_triangle(m, q, hint, best);
     uint32_t stack[kMeshStack];
     int sp = 0;

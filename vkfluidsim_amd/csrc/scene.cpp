@@ -334,23 +334,26 @@ vfs_params WCSPHWithBoundaryModel::RawParameters() const {
     return p;
 }
 
+vfs_boundary_object WCSPHWithBoundaryModel::BoundaryObjectRecord(size_t k) const {
+    const auto& o = objects.at(k);
+    vfs_boundary_object d{};
+    std::memcpy(d.transform, o.transform.data(), sizeof(d.transform));
+    std::memcpy(d.rotation, o.rotation.data(), sizeof(d.rotation));
+    for (int a = 0; a < 3; a++) {
+        d.box_size[a] = o.box.size[a];
+        d.box_pos[a] = o.box.pos[a];
+        d.resolution[a] = o.resolution[a];
+    }
+    d.sdf_grid = o.sdf.data();
+    d.volume_map_grid = o.volume_map.data();
+    return d;
+}
+
 void WCSPHWithBoundaryModel::Init() {
     Clear();
     const vfs_params p = RawParameters();
     std::vector<vfs_boundary_object> objs(objects.size());
-    for (size_t k = 0; k < objects.size(); k++) {
-        auto& o = objects[k];
-        auto& d = objs[k];
-        std::memcpy(d.transform, o.transform.data(), sizeof(d.transform));
-        std::memcpy(d.rotation, o.rotation.data(), sizeof(d.rotation));
-        for (int a = 0; a < 3; a++) {
-            d.box_size[a] = o.box.size[a];
-            d.box_pos[a] = o.box.pos[a];
-            d.resolution[a] = o.resolution[a];
-        }
-        d.sdf_grid = o.sdf.data();
-        d.volume_map_grid = o.volume_map.data();
-    }
+    for (size_t k = 0; k < objects.size(); k++) objs[k] = BoundaryObjectRecord(k);
     Check(vfs_sim_create(&p, objs.empty() ? nullptr : objs.data(), (uint32_t)objs.size(), &sim), "vfs_sim_create");
 }
 
@@ -732,6 +735,17 @@ int vfs_scene_reset(vfs_scene* scene) {
 int vfs_scene_step(vfs_scene* scene, void* stream) {
     if (!scene || !scene->scene) return vfs::fail(VFS_ERR_INVALID, "null argument");
     return guarded([&] { scene->scene->Step(reinterpret_cast<hipStream_t>(stream)); });
+}
+
+int vfs_scene_boundary_objects(vfs_scene* scene, vfs_boundary_object* out, uint32_t capacity, uint32_t* count) {
+    if (!scene || !scene->scene || !count) return vfs::fail(VFS_ERR_INVALID, "null argument");
+    auto* model = dynamic_cast<vfs::WCSPHWithBoundaryModel*>(scene->scene->GetModel());
+    if (!model) return vfs::fail(VFS_ERR_STATE, "the scene has no WCSPH model");
+    const auto& objs = model->GetBoundaryObjects();
+    *count = (uint32_t)objs.size();
+    if (out)
+        for (uint32_t k = 0; k < objs.size() && k < capacity; k++) out[k] = model->BoundaryObjectRecord(k);
+    return 0;
 }
 
 int vfs_scene_destroy(vfs_scene* scene) {
