@@ -135,6 +135,9 @@ template <class R>
 struct Model {
     // derived uniforms
     R g[3], h, h2, inv_h, cell, rho0, B, gamma, nu, kwall, dt, mass;
+    // the adjoint of an FP32 trajectory takes the forward's cells: float
+    // division of the (float) positions by the cell size
+    bool float_cells = false;
     R sigma;       // cubic_spline_scale (model.cpp:231)
     R grad_sigma;  // grad_cubic_spline_scale / smooth_radius (kernels_3d.slang:80-81)
     R bb_lo[3], bb_hi[3];
@@ -676,7 +679,12 @@ static void grid_density(const Model<R>& m, const std::vector<R>& x, uint32_t n,
 template <class R>
 static void stencil_keys(const Model<R>& m, const R* p, uint32_t n, uint32_t* k) {
     int32_t c[3];
-    get_cell<R>(p, m.cell, c);
+    if (m.float_cells) {
+        const float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
+        get_cell<float>(pf, (float)m.cell, c);
+    } else {
+        get_cell<R>(p, m.cell, c);
+    }
     for (int o = 0; o < 27; o++)
         k[o] = hash_cell(c[0] + kOffsets3D[o][0], c[1] + kOffsets3D[o][1], c[2] + kOffsets3D[o][2]) % n;
 }
@@ -1712,18 +1720,19 @@ int orc_adjoint_run(void* h, uint32_t k, const orc::GridC* grid, const double* t
 
 // The FP64 adjoint of the FP32 trajectory: the forward runs on the FP32
 // handle h32 (the trajectory the GPU reproduces bit for bit), its tape is
-// widened to double, the loss and its gradient are evaluated as the GPU
-// evaluates them (FP32 kernel values summed in double), and the reverse
-// substeps use the model of the FP64 handle h64 (same parameters and
-// objects; its state is unused).  Separates the reverse substeps' arithmetic
-// from the divergence of FP32 and FP64 trajectories.
+// widened to double, and the loss gradient and the adjoint use the model of
+// the FP64 handle h64 (same parameters and objects; its state is unused).
+// Separates the adjoint's arithmetic from the divergence of FP32 and FP64
+// trajectories.  The loss is evaluated in double (as the GPU evaluates it:
+// float positions and grid points, kernel values in double).
 int orc_adjoint_run_f32_trajectory(void* h32, void* h64, uint32_t k, const orc::GridC* grid, const double* target,
                                    double* loss, double* grad_x, double* grad_v, double* grad_nu) {
     auto* o32 = (OrcSim*)h32;
     auto* o64 = (OrcSim*)h64;
     if (o32->fp64 || !o64->fp64 || !grid) return -1;
     auto& sim = o32->s32;
-    const orc::Model<double>& m = o64->s64.m;
+    orc::Model<double> m = o64->s64.m;
+    m.float_cells = true;
     const uint32_t n = sim.n;
     std::vector<orc::Sim<float>::TapeStep> tape32;
     sim.tape = &tape32;
@@ -1742,31 +1751,10 @@ int orc_adjoint_run_f32_trajectory(void* h32, void* h64, uint32_t k, const orc::
         tape[s].nbr.swap(tape32[s].nbr);
     }
     tape32.clear();
-    // The loss and its gradient with the kernel evaluated as the GPU does: float
-    // positions, grid points, distances and kernel values (the FP32 model),
-    // summed in double.  The comparison then isolates the reverse substeps.
-    const orc::Model<float>& mf = sim.m;
-    const size_t G = (size_t)grid->dims[0] * grid->dims[1] * grid->dims[2];
-    std::vector<double> rg(G, 0.0);
-    auto for_pairs = [&](auto&& f) {
-        for (uint32_t j = 0; j < n; j++) {
-            const float* xj = &sim.x[3 * (size_t)j];
-            int lo[3], hi[3];
-            orc::grid_range<float>(*grid, xj, mf.h, lo, hi);
-            for (int iz = lo[2]; iz <= hi[2]; iz++)
-                for (int iy = lo[1]; iy <= hi[1]; iy++)
-                    for (int ix = lo[0]; ix <= hi[0]; ix++) {
-                        float gp[3], d[3];
-                        orc::grid_point<float>(*grid, ix, iy, iz, gp);
-                        for (int a = 0; a < 3; a++) d[a] = xj[a] - gp[a];
-                        const float r2 = orc::dot3<float>(d, d);
-                        if (r2 > mf.h2) continue;
-                        const size_t idx = (size_t)ix + grid->dims[0] * ((size_t)iy + (size_t)grid->dims[1] * iz);
-                        f(j, idx, d, std::sqrt(r2));
-                    }
-        }
-    };
-    for_pairs([&](uint32_t, size_t idx, const float*, float r) { rg[idx] += (double)(mf.mass * mf.W(r)); });
+    const std::vector<double> x = widen(sim.x);
+    std::vector<double> rg;
+    orc::grid_density<double>(m, x, n, *grid, rg);
+    const size_t G = rg.size();
     double L = 0.0;
     for (size_t g = 0; g < G; g++) {
         const double d = rg[g] - (target ? target[g] : 0.0);
@@ -1774,14 +1762,25 @@ int orc_adjoint_run_f32_trajectory(void* h32, void* h64, uint32_t k, const orc::
     }
     if (loss) *loss = L / (double)G;
     std::vector<double> gx(3 * (size_t)n, 0.0), gv(3 * (size_t)n, 0.0);
-    for_pairs([&](uint32_t j, size_t idx, const float* d, float r) {
-        if (!(r > 1e-9f)) return;
-        const double cg = 2.0 * (rg[idx] - (target ? target[idx] : 0.0)) / (double)G;
-        const double s2 = cg * (double)(mf.mass * (mf.h * mf.dW(r)) / r);
-        for (int a = 0; a < 3; a++) gx[3 * (size_t)j + a] += s2 * (double)d[a];
-    });
-    // the GPU hands the seed to the reverse substeps as float
-    for (double& g : gx) g = (double)(float)g;
+    for (uint32_t j = 0; j < n; j++) {
+        int lo[3], hi[3];
+        orc::grid_range<double>(*grid, &x[3 * j], m.h, lo, hi);
+        for (int iz = lo[2]; iz <= hi[2]; iz++)
+            for (int iy = lo[1]; iy <= hi[1]; iy++)
+                for (int ix = lo[0]; ix <= hi[0]; ix++) {
+                    double gp[3], d[3];
+                    orc::grid_point<double>(*grid, ix, iy, iz, gp);
+                    for (int a = 0; a < 3; a++) d[a] = x[3 * j + a] - gp[a];
+                    const double r2 = orc::dot3<double>(d, d);
+                    if (r2 > m.h2) continue;
+                    const double r = std::sqrt(r2);
+                    if (!(r > 1e-9)) continue;
+                    const size_t idx = (size_t)ix + grid->dims[0] * ((size_t)iy + (size_t)grid->dims[1] * iz);
+                    const double cg = 2.0 * (rg[idx] - (target ? target[idx] : 0.0)) / (double)G;
+                    const double s2 = cg * m.mass * (m.h * m.dW(r)) / r;
+                    for (int a = 0; a < 3; a++) gx[3 * j + a] += s2 * d[a];
+                }
+    }
     double nu_bar = 0.0;
     for (int s = (int)k - 1; s >= 0; s--) orc::adjoint_substep<double>(m, tape[s], n, gx, gv, nu_bar);
     for (size_t i = 0; i < 3 * (size_t)n; i++) {

@@ -44,10 +44,40 @@ def test_host_adjoint_matches_fp64_finite_differences():
             assert gx[i, a] == pytest.approx(fd, rel=1e-3, abs=1e-6 * np.abs(gx).max()), (i, a)
     dv = np.random.default_rng(7).standard_normal(v0.shape)
     fd = (loss(x0, v0 + 1e-6 * dv) - loss(x0, v0 - 1e-6 * dv)) / 2e-6
-    assert float(np.sum(gv * dv)) == pytest.approx(fd, rel=1e-3)
+    assert float(np.sum(gv * dv)) == pytest.approx(fd, rel=1e-5)
+    # the viscosity parameter is a float: difference over the float values taken
     nu = float(p.viscosity_strenght)
-    fd = (loss(x0, v0, nu + 1e-7) - loss(x0, v0, nu - 1e-7)) / 2e-7
-    assert gnu == pytest.approx(fd, rel=1e-2)
+    hi, lo = float(np.float32(nu + 1e-7)), float(np.float32(nu - 1e-7))
+    fd = (loss(x0, v0, hi) - loss(x0, v0, lo)) / (hi - lo)
+    assert gnu == pytest.approx(fd, rel=1e-3)
+
+
+def test_host_adjoint_matches_fp64_finite_differences_200_substeps():
+    """The 4,096-particle variant over 200 substeps: directional derivative
+    along random initial velocities and positions against FP64 central
+    differences."""
+    w = CF.dam_break(side=(16, 16, 16), map_res=32)
+    grids = [H.oracle_volume_map(k, sp, r, w.h) for k, sp, r in w.shapes]
+    p = V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos)
+    dims = (32, 32, 32)
+    spacing = tuple(b / (d - 1) for b, d in zip(w.box_size, dims))
+    x0 = w.x.astype(np.float64)
+    v_t = 0.01 * CF.normal(CF.SEED_BASE + 4, 3 * w.n).reshape(w.n, 3)
+    steps = 200
+    _, target, _, _, _ = H.oracle_adjoint(p, grids, x0, v_t, steps, (0, 0, 0), spacing, dims)
+    v0 = np.zeros_like(x0)
+    L, _, gx, gv, gnu = H.oracle_adjoint(p, grids, x0, v0, steps, (0, 0, 0), spacing, dims, target)
+
+    def loss(x, v):
+        return H.oracle_adjoint(p, grids, x, v, steps, (0, 0, 0), spacing, dims, target)[0]
+
+    rng = np.random.default_rng(7)
+    dv = rng.standard_normal(v0.shape)
+    fd = (loss(x0, v0 + 1e-6 * dv) - loss(x0, v0 - 1e-6 * dv)) / 2e-6
+    assert float(np.sum(gv * dv)) == pytest.approx(fd, rel=1e-5)
+    dx = rng.standard_normal(x0.shape)
+    fd = (loss(x0 + 1e-8 * dx, v0) - loss(x0 - 1e-8 * dx, v0)) / 2e-8
+    assert float(np.sum(gx * dx)) == pytest.approx(fd, rel=1e-3)
 
 
 def test_fp64_adjoint_of_fp32_trajectory_close_to_fp64_adjoint():

@@ -66,7 +66,8 @@ def test_adjoint_forward_matches_plain_substeps():
     np.testing.assert_array_equal(x_a, x_r)
     np.testing.assert_array_equal(v_a, v_r)
     want = _grid_density_f64(x_r, float(np.float32(w.h)), _mass(w.h), spacing, dims)
-    assert np.abs(rho - want).max() <= 1e-4 * np.abs(want).max()
+    # the GPU evaluates the kernel in double: only the float copy of rho rounds
+    assert np.abs(rho - want).max() <= 1e-7 * np.abs(want).max()
     # the loss is summed from the double-precision field; rho is its float copy
     assert loss == pytest.approx(np.mean((rho.astype(np.float64) - target) ** 2), rel=1e-4)
     assert loss > 0
@@ -80,13 +81,14 @@ def _directional(adj, x0, v0, target, dx, dv):
 
 def test_adjoint_velocity_gradient_matches_gpu_finite_differences():
     """Central differences (eps = 1e-3) of the GPU forward along a random
-    direction of the initial velocities.  The FP32 loss is a small difference
-    of densities (its signal is a few float32 ulps of particle motion), so both
-    the FP32 difference quotient and the FP32 adjoint sit within a few percent
-    of the FP64 value (1.40e-3 here: the FP32 adjoint 0.8% off, the FP32
-    difference quotient 3% off); the FP64 host adjoint matches FP64 central
-    differences to 1e-6 (test_adjoint_cpu.py)."""
-    w, grids, p, sim, adj, target, spacing, dims = _setup()
+    direction of the initial velocities, 200 substeps.  The difference
+    quotient of the FP32 forward carries the rounding of its trajectory: it
+    sits 0.3% from the adjoint of that trajectory (measured: -9.406e-2 against
+    -9.378e-2; 0.3-0.5% for eps 1e-3 ... 1e-2).  The adjoint itself equals the
+    FP64 adjoint of the same trajectory to 1e-6
+    (test_adjoint_matches_fp64_adjoint_of_the_same_trajectory), whose algorithm
+    matches FP64 central differences to 1e-5 (test_adjoint_cpu.py)."""
+    w, grids, p, sim, adj, target, spacing, dims = _setup(steps=200, every=20)
     x0, v0 = w.x.astype(np.float32), w.v.astype(np.float32)
     adj.forward(x0, v0, target)
     gx, gv, gnu = adj.backward()
@@ -95,7 +97,7 @@ def test_adjoint_velocity_gradient_matches_gpu_finite_differences():
     fd = _directional(adj, x0, v0, target, np.zeros_like(x0), dv)
     an = float(np.sum(gv.astype(np.float64) * dv))
     print(f"d loss/d v0 along a random direction: adjoint {an:.6e}, GPU central difference {fd:.6e}")
-    assert an == pytest.approx(fd, rel=5e-2)
+    assert an == pytest.approx(fd, rel=1e-2)
 
 
 @pytest.mark.parametrize("steps", [20, 200])
@@ -127,12 +129,12 @@ def test_adjoint_matches_fp64_adjoint_of_the_same_trajectory(steps):
                                                      target.astype(np.float64))
     ex = np.linalg.norm(gx - mx) / np.linalg.norm(mx)
     ev = np.linalg.norm(gv - mv) / np.linalg.norm(mv)
-    print(f"{steps} substeps, same trajectory: x0 {ex:.2e}, v0 {ev:.2e}, viscosity {gnu:.6e} vs {mnu:.6e}")
-    # the GPU sums FP32 kernel values in double, the oracle evaluates the kernel
-    # in double; the loss, a small difference of densities, magnifies that
-    assert loss == pytest.approx(L, rel=1e-3)
-    assert ex < 5e-3 and ev < 5e-3
-    assert gnu == pytest.approx(mnu, rel=2e-2)
+    print(f"{steps} substeps, same trajectory: x0 {ex:.2e}, v0 {ev:.2e}, viscosity {gnu:.8e} vs {mnu:.8e}")
+    # both evaluate the loss and the reverse substeps in double on the same
+    # float tape (measured: 2.5e-8, the rounding of the float gradients)
+    assert loss == pytest.approx(L, rel=1e-9)
+    assert ex < 1e-6 and ev < 1e-6
+    assert gnu == pytest.approx(mnu, rel=1e-6)
 
 
 def test_torch_autograd_function_wrapper():

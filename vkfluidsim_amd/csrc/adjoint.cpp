@@ -13,20 +13,21 @@
 namespace vfs {
 
 int launch_adjoint_substep(const uint32_t* perm, const float4* xs, const float4* vs, const float* rho,
-                           const float4* bnd, const uint32_t* nbr, const uint32_t* cnt, const float4* gx_in,
-                           const float4* gv_in, float4* abar, float4* gx, float4* gv, float* rbar, float4* gpsi,
-                           uint32_t* dup, float4* gx_out, float4* gv_out, double* nu_part, double* nu_bar,
-                           uint32_t n, const SimConsts& c, const DevObject* objs, hipStream_t s);
+                           const float4* bnd, const uint32_t* nbr, const uint32_t* cnt, const double4* gx_in,
+                           const double4* gv_in, double4* abar, double4* gx, double4* gv, double* rbar,
+                           double4* gpsi, uint32_t* dup, double4* gx_out, double4* gv_out, double* nu_part,
+                           double* nu_bar, uint32_t n, const SimConsts& c, const DevObject* objs, hipStream_t s);
 int launch_grid_density(const float* origin, const float* spacing, const uint32_t* dims, const float4* xs,
                         const uint32_t* cell_start, const SimConsts& c, double* out, hipStream_t s);
 int launch_grid_loss_grad(const float* origin, const float* spacing, const uint32_t* dims, const float4* x,
-                          uint32_t n, const double* rho_g, const float* target, const SimConsts& c, float4* gx,
+                          uint32_t n, const double* rho_g, const float* target, const SimConsts& c, double4* gx,
                           hipStream_t s);
 int launch_d_to_f(const double* src, float* dst, uint32_t n, hipStream_t s);
 int launch_max_u32(const uint32_t* a, uint32_t n, uint32_t* out, hipStream_t s);
 int launch_f3_to_f4(const float* src, float4* dst, uint32_t n, hipStream_t s);
 int launch_f4_to_f3(const float4* src, float* dst, uint32_t n, hipStream_t s);
-int launch_zero_f4(float4* a, uint32_t n, hipStream_t s);
+int launch_zero_d4(double4* a, uint32_t n, hipStream_t s);
+int launch_d4_to_f3(const double4* src, float* dst, uint32_t n, hipStream_t s);
 
 }  // namespace vfs
 
@@ -44,9 +45,10 @@ struct vfs_adjoint {
         uint32_t *nbr = nullptr, *cnt = nullptr;
     };
     std::vector<Slot> tape;
-    float4 *gx_a = nullptr, *gv_a = nullptr, *gx_b = nullptr, *gv_b = nullptr;
-    float4 *abar = nullptr, *gx = nullptr, *gv = nullptr, *gpsi = nullptr;
-    float* rbar = nullptr;
+    // adjoint state in double (the reverse pass accumulates over every substep)
+    double4 *gx_a = nullptr, *gv_a = nullptr, *gx_b = nullptr, *gv_b = nullptr;
+    double4 *abar = nullptr, *gx = nullptr, *gv = nullptr, *gpsi = nullptr;
+    double* rbar = nullptr;
     uint32_t *dup = nullptr, *maxc = nullptr;
     double *nu_bar = nullptr, *nu_part = nullptr;
     double* rho_grid = nullptr;  // sampled density, summed in double
@@ -241,9 +243,9 @@ int vfs_adjoint_backward(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev,
     // d loss / d x of the final state (its array order), zero velocity adjoint
     VFS_TRY(launch_grid_loss_grad(a->origin, a->spacing, a->dims, a->ck_x[n_seg], n, a->rho_grid, a->target, c,
                                   a->gx_a, s));
-    VFS_TRY(launch_zero_f4(a->gv_a, n, s));
+    VFS_TRY(launch_zero_d4(a->gv_a, n, s));
     VFS_HIP(hipMemsetAsync(a->nu_bar, 0, sizeof(double), s));
-    float4 *gx_cur = a->gx_a, *gv_cur = a->gv_a, *gx_nxt = a->gx_b, *gv_nxt = a->gv_b;
+    double4 *gx_cur = a->gx_a, *gv_cur = a->gv_a, *gx_nxt = a->gx_b, *gv_nxt = a->gv_b;
     for (int seg = (int)n_seg - 1; seg >= 0; seg--) {
         const uint32_t first = (uint32_t)seg * a->C;
         const uint32_t steps = std::min(a->C, a->T - first);
@@ -268,8 +270,8 @@ int vfs_adjoint_backward(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev,
             std::swap(gv_cur, gv_nxt);
         }
     }
-    if (grad_x0_dev) VFS_TRY(launch_f4_to_f3(gx_cur, grad_x0_dev, n, s));
-    if (grad_v0_dev) VFS_TRY(launch_f4_to_f3(gv_cur, grad_v0_dev, n, s));
+    if (grad_x0_dev) VFS_TRY(launch_d4_to_f3(gx_cur, grad_x0_dev, n, s));
+    if (grad_v0_dev) VFS_TRY(launch_d4_to_f3(gv_cur, grad_v0_dev, n, s));
     double nb = 0.0;
     VFS_HIP(hipMemcpyAsync(&nb, a->nu_bar, sizeof(double), hipMemcpyDeviceToHost, s));
     // leave the simulation at the forward's final state

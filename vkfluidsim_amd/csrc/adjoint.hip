@@ -12,6 +12,8 @@
 // equally often unless the 27 stencil keys of one of the two particles repeat
 // (hash collisions); then the neighbour's term is weighted by the ratio of the
 // two multiplicities, counted from the stencil keys.
+#include <cmath>
+
 #include "wcsph_math.h"
 
 namespace vfs {
@@ -48,6 +50,87 @@ __device__ __forceinline__ float eos_dp(float rho, const SimConsts& c) {
     return (float)((double)c.B * (double)c.gamma * xp / (double)c.rho0);
 }
 
+// Constants of the reverse pair terms in double, from the model's float
+// parameters (the pair sums cancel strongly, so their terms are evaluated in
+// double from the float tape; see AdjConsts::make).
+struct AdjConsts {
+    double h, inv_h, h2, sigma, grad_sigma, mass, nu10, visc_eps_h2, rho0, B, gamma, dt, kwall_neg;
+    int32_t gamma_int;
+};
+
+__device__ __forceinline__ double w_d(double r, const AdjConsts& k) {
+    const double q = r * k.inv_h;
+    if (q <= 1.0) {
+        if (q > 0.5) {
+            const double t = 1.0 - q;
+            return (k.sigma * 2.0) * ((t * t) * t);
+        }
+        const double q2 = q * q;
+        const double q3 = q2 * q;
+        return k.sigma * ((6.0 * q3 - 6.0 * q2) + 1.0);
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ double dw_d(double r, const AdjConsts& k) {
+    const double q = r * k.inv_h;
+    if ((r > (double)1e-9f) && (q <= 1.0)) {
+        if (q > 0.5) {
+            const double t = 1.0 - q;
+            return (-k.grad_sigma) * (t * t);
+        }
+        return (k.grad_sigma * q) * (3.0 * q - 2.0);
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ double ddw_d(double r, const AdjConsts& k) {
+    const double q = r * k.inv_h;
+    if ((r > (double)1e-9f) && (q <= 1.0)) {
+        if (q > 0.5) return ((2.0 * k.grad_sigma) * (1.0 - q)) * k.inv_h;
+        return (k.grad_sigma * (6.0 * q - 2.0)) * k.inv_h;
+    }
+    return 0.0;
+}
+
+// (rho / rho0)^e in double: binary powering for an integral exponent.
+__device__ __forceinline__ double pow_ratio_d(double rho, int32_t e_int, double e, const AdjConsts& k) {
+    const double x = rho / k.rho0;
+    if (e_int >= 0) {
+        unsigned n = (unsigned)e_int;
+        double base = x, r = 1.0;
+        while (n) {
+            if (n & 1u) r *= base;
+            base *= base;
+            n >>= 1;
+        }
+        return r;
+    }
+    return pow(x, e);
+}
+
+__device__ __forceinline__ double eos_d(double rho, const AdjConsts& k) {
+    return k.B * (pow_ratio_d(fmax(rho, k.rho0), k.gamma_int, k.gamma, k) - 1.0);
+}
+
+__device__ __forceinline__ double eos_dp_d(double rho, const AdjConsts& k) {
+    if (!(rho > k.rho0)) return 0.0;
+    return k.B * k.gamma * pow_ratio_d(rho, k.gamma_int >= 1 ? k.gamma_int - 1 : -1, k.gamma - 1.0, k) / k.rho0;
+}
+
+// p / rho^2 and m / max(rho, 1e-6) of a particle, in double from its float density.
+__device__ __forceinline__ double pd_d(float rho, const AdjConsts& k) {
+    const double r = rho;
+    return rho > 0.0f ? eos_d(r, k) / (r * r) : 0.0;
+}
+__device__ __forceinline__ double md_d(float rho, const AdjConsts& k) {
+    return k.mass / fmax((double)rho, (double)1e-6f);
+}
+
+__device__ __forceinline__ double dot3d(double ax, double ay, double az, double bx, double by, double bz) {
+    return fma(az, bz, fma(ay, by, ax * bx));
+}
+
 __device__ __forceinline__ uint32_t own_key(float x, float y, float z, const SimConsts& c) {
     return cell_key(floor_to_cell(x / c.cell), floor_to_cell(y / c.cell), floor_to_cell(z / c.cell), 0, 0, 0, c);
 }
@@ -80,135 +163,170 @@ __device__ __forceinline__ float pair_weight(const float4& xi, const float4& xj,
     return m_ij ? (float)m_ji / (float)m_ij : 0.0f;
 }
 
-// d(trilinear value)/d(uvw) for corners g and cell coordinates chi.
-__device__ __forceinline__ void grid_grad(const float* g, const float* chi, const uint32_t* res, float* out) {
-    const float f0[3] = {1.0f - chi[0], 1.0f - chi[1], 1.0f - chi[2]};
-    const float f1[3] = {1.0f + chi[0], 1.0f + chi[1], 1.0f + chi[2]};
-    // corner k has offsets (ox, oy, oz) = (k >> 2, k & 1, (k >> 1) & 1)
-    float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+// SampleGrid's cell, value and d(value)/d(uvw) in double (base cell clamped
+// to [0, n-2] as in the forward), on float grids.
+__device__ __forceinline__ uint32_t grid_cell_d(const uint32_t* res, const double* uvw, double* chi) {
+    uint32_t ci[3];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int ox = k >> 2, oy = k & 1, oz = (k >> 1) & 1;
-        const float fx = ox ? f1[0] : f0[0], fy = oy ? f1[1] : f0[1], fz = oz ? f1[2] : f0[2];
-        const float sx = ox ? 1.0f : -1.0f, sy = oy ? 1.0f : -1.0f, sz = oz ? 1.0f : -1.0f;
-        dx += g[k] * (sx * fy * fz);
-        dy += g[k] * (fx * sy * fz);
-        dz += g[k] * (fx * fy * sz);
+    for (int a = 0; a < 3; a++) {
+        const double x = ((double)res[a] - 1.0) * uvw[a];
+        const double f = floor(x);
+        const uint32_t hi = res[a] >= 2u ? res[a] - 2u : 0u;
+        const uint32_t cc = f >= 0.0 ? (f <= (double)hi ? (uint32_t)f : hi) : 0u;
+        ci[a] = cc;
+        chi[a] = 2.0 * (x - (double)cc) - 1.0;
     }
-    out[0] = dx * (2.0f * ((float)res[0] - 1.0f)) / 8.0f;
-    out[1] = dy * (2.0f * ((float)res[1] - 1.0f)) / 8.0f;
-    out[2] = dz * (2.0f * ((float)res[2] - 1.0f)) / 8.0f;
+    return ci[0] + res[0] * ci[1] + (res[0] * res[1]) * ci[2];
 }
 
-// Adjoint of CalculateBoundaryVolume (boundary_volume) at `pos`: adds to g
+__device__ __forceinline__ void grid_corners_d(const float* __restrict__ grid, const uint32_t* res, uint32_t b0,
+                                               double* g) {
+    float f[8];
+    grid_corners(grid, res, b0, f);
+#pragma unroll
+    for (int q = 0; q < 8; q++) g[q] = f[q];
+}
+
+// value and gradient with respect to uvw; corner q has offsets
+// (ox, oy, oz) = (q >> 2, q & 1, (q >> 1) & 1)
+__device__ __forceinline__ double grid_value_grad_d(const double* g, const double* chi, const uint32_t* res,
+                                                    double* out) {
+    const double f0[3] = {1.0 - chi[0], 1.0 - chi[1], 1.0 - chi[2]};
+    const double f1[3] = {1.0 + chi[0], 1.0 + chi[1], 1.0 + chi[2]};
+    double v = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int ox = q >> 2, oy = q & 1, oz = (q >> 1) & 1;
+        const double fx = ox ? f1[0] : f0[0], fy = oy ? f1[1] : f0[1], fz = oz ? f1[2] : f0[2];
+        const double sx = ox ? 1.0 : -1.0, sy = oy ? 1.0 : -1.0, sz = oz ? 1.0 : -1.0;
+        v += g[q] * ((fx * fy) * fz);
+        dx += g[q] * (sx * fy * fz);
+        dy += g[q] * (fx * sy * fz);
+        dz += g[q] * (fx * fy * sz);
+    }
+    out[0] = dx * (2.0 * ((double)res[0] - 1.0)) / 8.0;
+    out[1] = dy * (2.0 * ((double)res[1] - 1.0)) / 8.0;
+    out[2] = dz * (2.0 * ((double)res[2] - 1.0)) / 8.0;
+    return v / 8.0;
+}
+
+// The SDF value and gradient (d/d object-space position) at q, in double.
+__device__ __forceinline__ double sdf_value_grad_d(const DevObject& o, const double* q, double* grad) {
+    double u[3], ch[3], g8[8];
+#pragma unroll
+    for (int a = 0; a < 3; a++) u[a] = (q[a] - (double)o.box_pos[a]) / (double)o.box_size[a];
+    const uint32_t cq = grid_cell_d(o.res, u, ch);
+    grid_corners_d(o.sdf, o.res, cq, g8);
+    const double v = grid_value_grad_d(g8, ch, o.res, grad);
+#pragma unroll
+    for (int a = 0; a < 3; a++) grad[a] /= (double)o.box_size[a];
+    return v;
+}
+
+// Adjoint of CalculateBoundaryVolume (boundary_volume) at `pos`: adds to gpos
 // the derivative of psi_bar * psi + grad_bar . grad psi with respect to pos,
-// on the branch the forward took (grid cells held fixed).
-__device__ void boundary_volume_adjoint(const DevObject* __restrict__ objs, uint32_t n_obj, const float* pos,
-                                        const SimConsts& c, float psi_bar, const float* grad_bar, float* gpos) {
+// on the branch the forward took (grid cells held fixed), in double (the
+// normal's Jacobian is a difference of nearby grid gradients).
+__device__ void boundary_volume_adjoint(const DevObject* __restrict__ objs, uint32_t n_obj, const float* pos_f,
+                                        const AdjConsts& k, double psi_bar, const double* grad_bar, double* gpos) {
+    const double pos[3] = {pos_f[0], pos_f[1], pos_f[2]};
     for (uint32_t oi = 0; oi < n_obj; oi++) {
         const DevObject& o = objs[oi];
-        float lp[3];
-        mat_vec(o.T, pos, lp);
+        double lp[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            lp[r] = (((double)o.T[r] * pos[0] + (double)o.T[4 + r] * pos[1]) + (double)o.T[8 + r] * pos[2]) +
+                    (double)o.T[12 + r];
         bool outside = false;
 #pragma unroll
         for (int a = 0; a < 3; a++)
-            outside |= (lp[a] < o.box_pos[a]) || (lp[a] > o.box_pos[a] + o.box_size[a]);
+            outside |= (lp[a] < (double)o.box_pos[a]) || (lp[a] > (double)o.box_pos[a] + (double)o.box_size[a]);
         if (outside) continue;
-        const float inv_bs[3] = {1.0f / o.box_size[0], 1.0f / o.box_size[1], 1.0f / o.box_size[2]};
-        float uvw[3], chi[3], gs[8], gv[8];
-        to_uvw(o, lp, uvw);
-        const uint32_t cell = grid_cell(o.res, uvw, chi);
-        grid_corners(o.sdf, o.res, cell, gs);
-        const float dist = grid_interp(gs, chi);
-        if (!(dist >= 0.0f && dist < c.h)) continue;
-        grid_corners(o.vm, o.res, cell, gv);
-        const float volume = grid_interp(gv, chi);
-        if (!(volume > 0.0f)) continue;
-        float gS[3], gV[3];
-        grid_grad(gs, chi, o.res, gS);
-        grid_grad(gv, chi, o.res, gV);
+        double uvw[3], chi[3], gs[8], gv[8], gS[3], gV[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) uvw[a] = (lp[a] - (double)o.box_pos[a]) / (double)o.box_size[a];
+        const uint32_t cell = grid_cell_d(o.res, uvw, chi);
+        grid_corners_d(o.sdf, o.res, cell, gs);
+        const double dist = grid_value_grad_d(gs, chi, o.res, gS);
+        if (!(dist >= 0.0 && dist < k.h)) continue;
+        grid_corners_d(o.vm, o.res, cell, gv);
+        const double volume = grid_value_grad_d(gv, chi, o.res, gV);
+        if (!(volume > 0.0)) continue;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            gS[a] *= inv_bs[a];
-            gV[a] *= inv_bs[a];
+            gS[a] /= (double)o.box_size[a];
+            gV[a] /= (double)o.box_size[a];
         }
         // central-difference normal and its Jacobian Jn[a][b] = d nrm_a / d lp_b
-        const float eps = 0.1f * c.h;
-        float nrm[3], Jn[3][3];
+        const double eps = (double)0.1f * k.h;
+        double nrm[3], Jn[3][3];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            float v[2], gr[2][3];
+            double qp[3] = {lp[0], lp[1], lp[2]}, qm[3] = {lp[0], lp[1], lp[2]};
+            qp[a] = lp[a] + eps;
+            qm[a] = lp[a] - eps;
+            double gp[3], gm[3];
+            const double vp = sdf_value_grad_d(o, qp, gp), vm = sdf_value_grad_d(o, qm, gm);
+            nrm[a] = (vp - vm) / (2.0 * eps);
 #pragma unroll
-            for (int sgn = 0; sgn < 2; sgn++) {
-                float q[3] = {lp[0], lp[1], lp[2]};
-                q[a] = sgn == 0 ? lp[a] + eps : lp[a] - eps;
-                float u[3], ch[3], g8[8];
-                to_uvw(o, q, u);
-                const uint32_t cq = grid_cell(o.res, u, ch);
-                grid_corners(o.sdf, o.res, cq, g8);
-                v[sgn] = grid_interp(g8, ch);
-                grid_grad(g8, ch, o.res, gr[sgn]);
-            }
-            nrm[a] = (v[0] - v[1]) * (1.0f / (2.0f * eps));
-#pragma unroll
-            for (int b2 = 0; b2 < 3; b2++) Jn[a][b2] = (gr[0][b2] - gr[1][b2]) * inv_bs[b2] / (2.0f * eps);
+            for (int b2 = 0; b2 < 3; b2++) Jn[a][b2] = (gp[b2] - gm[b2]) / (2.0 * eps);
         }
-        const float modn = sqrtf(dot3(nrm[0], nrm[1], nrm[2], nrm[0], nrm[1], nrm[2]));
-        if (!(modn > 1e-9f)) continue;
+        const double modn = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+        if (!(modn > (double)1e-9f)) continue;
         // rn = (R nrm + t) / modn
-        float Rn[3], RJ[3][3];
+        double Rn[3], RJ[3][3], nJ[3];
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-            Rn[r] = o.Rm[r] * nrm[0] + o.Rm[4 + r] * nrm[1] + o.Rm[8 + r] * nrm[2] + o.Rm[12 + r];
+            Rn[r] = (double)o.Rm[r] * nrm[0] + (double)o.Rm[4 + r] * nrm[1] + (double)o.Rm[8 + r] * nrm[2] +
+                    (double)o.Rm[12 + r];
 #pragma unroll
             for (int b2 = 0; b2 < 3; b2++)
-                RJ[r][b2] = o.Rm[r] * Jn[0][b2] + o.Rm[4 + r] * Jn[1][b2] + o.Rm[8 + r] * Jn[2][b2];
+                RJ[r][b2] = (double)o.Rm[r] * Jn[0][b2] + (double)o.Rm[4 + r] * Jn[1][b2] + (double)o.Rm[8 + r] * Jn[2][b2];
         }
-        const float inv_m = 1.0f / modn;
-        float nJ[3];  // nrm^T Jn
 #pragma unroll
         for (int b2 = 0; b2 < 3; b2++) nJ[b2] = nrm[0] * Jn[0][b2] + nrm[1] * Jn[1][b2] + nrm[2] * Jn[2][b2];
-        float rn[3], Jr[3][3];
+        double rn[3], Jr[3][3];
 #pragma unroll
         for (int r = 0; r < 3; r++) {
-            rn[r] = Rn[r] * inv_m;
+            rn[r] = Rn[r] / modn;
 #pragma unroll
-            for (int b2 = 0; b2 < 3; b2++) Jr[r][b2] = RJ[r][b2] * inv_m - Rn[r] * nJ[b2] * (inv_m * inv_m * inv_m);
+            for (int b2 = 0; b2 < 3; b2++) Jr[r][b2] = RJ[r][b2] / modn - Rn[r] * nJ[b2] / (modn * modn * modn);
         }
-        const float dd = dist + 0.125f * c.h;
-        const float d = fmaxf(dd, 0.5f * c.h);
-        const bool d_follows = dd > 0.5f * c.h;
+        const double dd = dist + (double)0.125f * k.h;
+        const double d = fmax(dd, (double)0.5f * k.h);
+        const bool d_follows = dd > (double)0.5f * k.h;
         // xij = d rn, Jx = rn gd^T + d Jr
-        float xij[3], Jx[3][3];
+        double xij[3], Jx[3][3];
 #pragma unroll
         for (int r = 0; r < 3; r++) {
             xij[r] = d * rn[r];
 #pragma unroll
-            for (int b2 = 0; b2 < 3; b2++) Jx[r][b2] = (d_follows ? rn[r] * gS[b2] : 0.0f) + d * Jr[r][b2];
+            for (int b2 = 0; b2 < 3; b2++) Jx[r][b2] = (d_follows ? rn[r] * gS[b2] : 0.0) + d * Jr[r][b2];
         }
-        const float xm = sqrtf(dot3(xij[0], xij[1], xij[2], xij[0], xij[1], xij[2]));
-        if (!(xm > 0.0f)) continue;
-        const float xh[3] = {xij[0] / xm, xij[1] / xm, xij[2] / xm};
-        float dxm[3];  // x^T Jx
+        const double xm = sqrt(xij[0] * xij[0] + xij[1] * xij[1] + xij[2] * xij[2]);
+        if (!(xm > 0.0)) continue;
+        const double xh[3] = {xij[0] / xm, xij[1] / xm, xij[2] / xm};
+        double dxm[3];  // x^T Jx
 #pragma unroll
         for (int b2 = 0; b2 < 3; b2++) dxm[b2] = xh[0] * Jx[0][b2] + xh[1] * Jx[1][b2] + xh[2] * Jx[2][b2];
-        const float W = kernel_w(xm, c), G = kernel_dw(xm, c), Gp = kernel_ddw(xm, c);
-        const float Wp = c.h * G;
-        const float xg = xh[0] * grad_bar[0] + xh[1] * grad_bar[1] + xh[2] * grad_bar[2];
-        float perp[3];  // (I - x x^T) grad_bar / xm
+        const double W = w_d(xm, k), G = dw_d(xm, k), Gp = ddw_d(xm, k);
+        const double Wp = k.h * G;
+        const double xg = xh[0] * grad_bar[0] + xh[1] * grad_bar[1] + xh[2] * grad_bar[2];
+        double perp[3];  // (I - x x^T) grad_bar / xm
 #pragma unroll
         for (int r = 0; r < 3; r++) perp[r] = (grad_bar[r] - xg * xh[r]) / xm;
-        float glp[3];
+        double glp[3];
 #pragma unroll
         for (int b2 = 0; b2 < 3; b2++) {
-            const float jp = Jx[0][b2] * perp[0] + Jx[1][b2] * perp[1] + Jx[2][b2] * perp[2];
+            const double jp = Jx[0][b2] * perp[0] + Jx[1][b2] * perp[1] + Jx[2][b2] * perp[2];
             glp[b2] = psi_bar * (W * gV[b2] + volume * Wp * dxm[b2]) + xg * (G * gV[b2] + volume * Gp * dxm[b2]) +
                       volume * G * jp;
         }
         // lp = T pos + t
 #pragma unroll
         for (int b2 = 0; b2 < 3; b2++)
-            gpos[b2] += o.T[b2 * 4 + 0] * glp[0] + o.T[b2 * 4 + 1] * glp[1] + o.T[b2 * 4 + 2] * glp[2];
+            gpos[b2] += (double)o.T[b2 * 4 + 0] * glp[0] + (double)o.T[b2 * 4 + 1] * glp[1] +
+                        (double)o.T[b2 * 4 + 2] * glp[2];
     }
 }
 
@@ -223,65 +341,67 @@ struct TapeView {
 };
 
 struct AdjBuffers {
-    const float4* gx_in;  // adjoint of the substep's output positions (sorted order)
-    const float4* gv_in;
-    float4* abar;         // adjoint of the acceleration
-    float4* gx;           // adjoint of the substep's sorted input positions
-    float4* gv;
-    float* rbar;          // adjoint of the densities
-    float4* gpsi;         // adjoint of grad psi
-    uint32_t* dup;        // 1: the particle's stencil keys repeat
-    float4* gx_out;       // gx, gv in the order of the substep's input arrays
-    float4* gv_out;
-    double* nu_part;      // per-block viscosity adjoint of this substep
+    const double4* gx_in;  // adjoint of the substep's output positions (sorted order)
+    const double4* gv_in;
+    double4* abar;         // adjoint of the acceleration
+    double4* gx;           // adjoint of the substep's sorted input positions
+    double4* gv;
+    double* rbar;          // adjoint of the densities
+    double4* gpsi;         // adjoint of grad psi
+    uint32_t* dup;         // 1: the particle's stencil keys repeat
+    double4* gx_out;       // gx, gv in the order of the substep's input arrays
+    double4* gv_out;
+    double* nu_part;       // per-block viscosity adjoint of this substep
     uint32_t n;
 };
 
 // Symplectic Euler and wall penalty backwards; stencil-key repeats.
-__global__ __launch_bounds__(256) void k_adj_integrate(TapeView t, AdjBuffers a, SimConsts c) {
+__global__ __launch_bounds__(256) void k_adj_integrate(TapeView t, AdjBuffers a, SimConsts c, AdjConsts k) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= a.n) return;
-    const float4 gxo = a.gx_in[i], gvo = a.gv_in[i];
-    const float vt[3] = {gvo.x + c.dt * gxo.x, gvo.y + c.dt * gxo.y, gvo.z + c.dt * gxo.z};
-    const float ab[3] = {c.dt * vt[0], c.dt * vt[1], c.dt * vt[2]};
+    const double4 gxo = a.gx_in[i], gvo = a.gv_in[i];
+    const double vt[3] = {gvo.x + k.dt * gxo.x, gvo.y + k.dt * gxo.y, gvo.z + k.dt * gxo.z};
+    const double ab[3] = {k.dt * vt[0], k.dt * vt[1], k.dt * vt[2]};
     const float4 x4 = t.xs[i];
     const float p[3] = {x4.x, x4.y, x4.z};
-    float gx[3] = {gxo.x, gxo.y, gxo.z};
+    double gx[3] = {gxo.x, gxo.y, gxo.z};
     float diff[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) diff[k] = fminf(p[k] - c.bb_lo[k], 0.0f) - fminf(c.bb_hi[k] - p[k], 0.0f);
+    for (int q = 0; q < 3; q++) diff[q] = fminf(p[q] - c.bb_lo[q], 0.0f) - fminf(c.bb_hi[q] - p[q], 0.0f);
     const bool wall = fabsf(diff[0]) > 1e-9f || fabsf(diff[1]) > 1e-9f || fabsf(diff[2]) > 1e-9f;
     if (wall) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const float D = (p[k] < c.bb_lo[k] ? 1.0f : 0.0f) + (p[k] > c.bb_hi[k] ? 1.0f : 0.0f);
-            gx[k] += c.kwall_neg * D * ab[k];
+        for (int q = 0; q < 3; q++) {
+            const double D = (p[q] < c.bb_lo[q] ? 1.0 : 0.0) + (p[q] > c.bb_hi[q] ? 1.0 : 0.0);
+            gx[q] += k.kwall_neg * D * ab[q];
         }
     }
-    a.abar[i] = make_float4(ab[0], ab[1], ab[2], 0.0f);
-    a.gx[i] = make_float4(gx[0], gx[1], gx[2], 0.0f);
-    a.gv[i] = make_float4(vt[0], vt[1], vt[2], 0.0f);
-    uint32_t k[27];
-    stencil_keys(p[0], p[1], p[2], c, k);
+    a.abar[i] = make_double4(ab[0], ab[1], ab[2], 0.0);
+    a.gx[i] = make_double4(gx[0], gx[1], gx[2], 0.0);
+    a.gv[i] = make_double4(vt[0], vt[1], vt[2], 0.0);
+    uint32_t sk[27];
+    stencil_keys(p[0], p[1], p[2], c, sk);
     uint32_t dup = 0;
     for (int u = 1; u < 27 && !dup; u++)
-        for (int w = 0; w < u; w++) dup |= k[u] == k[w] ? 1u : 0u;
+        for (int w = 0; w < u; w++) dup |= sk[u] == sk[w] ? 1u : 0u;
     a.dup[i] = dup;
 }
 
 // Pressure, viscosity and boundary pressure backwards: adjoints of x, v (pair
 // terms), of the EOS terms and so of rho, of grad psi, and of the viscosity.
-__global__ __launch_bounds__(256) void k_adj_forces(TapeView t, AdjBuffers a, SimConsts c) {
+// The pair terms are evaluated in double from the float tape (positions,
+// velocities, densities): the per-particle sums cancel strongly.
+__global__ __launch_bounds__(256) void k_adj_forces(TapeView t, AdjBuffers a, SimConsts c, AdjConsts k) {
     __shared__ double s_nu[256 / 64];
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     double nu_local = 0.0;
     if (i < a.n) {
         const float4 xi = t.xs[i], vi = t.vs[i];
-        const float Pi = xi.w, Mi = vi.w;
-        const float4 abi = a.abar[i];
+        const float rhoi = t.rho[i];
+        const double Pi = pd_d(rhoi, k), Mi = md_d(rhoi, k);
+        const double4 abi = a.abar[i];
         const bool dup_i = a.dup[i] != 0u;
-        const float ai[3] = {abi.x, abi.y, abi.z};
-        // per-particle sums of large, cancelling pair terms: double accumulators
+        const double ai[3] = {abi.x, abi.y, abi.z};
         double Pbar = 0.0, Mbar = 0.0, gx[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
         double fv[3] = {0.0, 0.0, 0.0};
         const uint32_t count = t.cnt[i];
@@ -290,79 +410,69 @@ __global__ __launch_bounds__(256) void k_adj_forces(TapeView t, AdjBuffers a, Si
             const uint32_t j = list[(size_t)(e >> 2) * 4u * a.n + 4u * i + (e & 3u)];
             if (j == i) continue;
             const float4 xj = t.xs[j], vj = t.vs[j];
-            const float xij[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
-            const float r2 = dot3(xij[0], xij[1], xij[2], xij[0], xij[1], xij[2]);
-            const float r = sqrtf(r2);
-            if (!(r > 1e-9f)) continue;
-            const float inv_r = 1.0f / r;
-            const float nh[3] = {xij[0] * inv_r, xij[1] * inv_r, xij[2] * inv_r};
-            const float G = kernel_dw(r, c), Gp = kernel_ddw(r, c);
-            const float F[3] = {nh[0] * G, nh[1] * G, nh[2] * G};
-            const float vij[3] = {vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
-            const float sig = dot3(vij[0], vij[1], vij[2], xij[0], xij[1], xij[2]);
-            const float inv_den = 1.0f / (r2 + c.visc_eps_h2);
-            const float Pj = xj.w, Mj = vj.w;
-            const float4 abj4 = a.abar[j];
-            const float aj[3] = {abj4.x, abj4.y, abj4.z};
-            const float w = pair_weight(xi, xj, dup_i, a.dup[j] != 0u, c);
+            const double xij[3] = {(double)xi.x - xj.x, (double)xi.y - xj.y, (double)xi.z - xj.z};
+            const double r2 = dot3d(xij[0], xij[1], xij[2], xij[0], xij[1], xij[2]);
+            const double r = sqrt(r2);
+            if (!(r > (double)1e-9f)) continue;
+            const double inv_r = 1.0 / r;
+            const double nh[3] = {xij[0] * inv_r, xij[1] * inv_r, xij[2] * inv_r};
+            const double G = dw_d(r, k), Gp = ddw_d(r, k);
+            const double F[3] = {nh[0] * G, nh[1] * G, nh[2] * G};
+            const double vij[3] = {(double)vi.x - vj.x, (double)vi.y - vj.y, (double)vi.z - vj.z};
+            const double sig = dot3d(vij[0], vij[1], vij[2], xij[0], xij[1], xij[2]);
+            const double inv_den = 1.0 / (r2 + k.visc_eps_h2);
+            const float rhoj = t.rho[j];
+            const double Pj = pd_d(rhoj, k), Mj = md_d(rhoj, k);
+            const double4 abj4 = a.abar[j];
+            const double aj[3] = {abj4.x, abj4.y, abj4.z};
+            const double w = pair_weight(xi, xj, dup_i, a.dup[j] != 0u, c);
             // forward viscous sum (for the viscosity adjoint)
 #pragma unroll
-            for (int k = 0; k < 3; k++) fv[k] += Mj * sig * inv_den * F[k];
+            for (int q = 0; q < 3; q++) fv[q] += Mj * sig * inv_den * F[q];
             // own term of a_i: -m (Pi + Pj) F + nu10 Mj sig inv_den F
-            const float aF = ai[0] * F[0] + ai[1] * F[1] + ai[2] * F[2];
-            Pbar += -c.mass * aF;
-            float Fb[3];
-#pragma unroll
-            for (int k = 0; k < 3; k++) Fb[k] = -c.mass * (Pi + Pj) * ai[k] + c.nu10 * Mj * sig * inv_den * ai[k];
-            float sb = c.nu10 * Mj * inv_den * aF;
-            float db = -c.nu10 * Mj * sig * inv_den * inv_den * aF;
+            const double aF = ai[0] * F[0] + ai[1] * F[1] + ai[2] * F[2];
             // neighbour term of a_j (its pair with i): F_ji = -F, x_ji = -x_ij, v_ji = -v_ij
-            const float bF = -(aj[0] * F[0] + aj[1] * F[1] + aj[2] * F[2]);  // a_j . F_ji
-            Pbar += w * (-c.mass * bF);
-            Mbar += w * (c.nu10 * sig * inv_den * bF);
-            // d(a_j term)/d x_i = -d/d x_ji; fold it into the own-term adjoint by
-            // expressing it in x_ij: F_ji(x_ji) = -F(x_ij), sig and den even
-            float Fb2[3];
+            const double bF = -(aj[0] * F[0] + aj[1] * F[1] + aj[2] * F[2]);  // a_j . F_ji
+            Pbar += -k.mass * aF + w * (-k.mass * bF);
+            Mbar += w * (k.nu10 * sig * inv_den * bF);
+            // d(a_j term)/d x_i = -d/d x_ji, folded into the own-term adjoint in
+            // x_ij: F_ji(x_ji) = -F(x_ij), sig and den even
+            double Fb[3];
 #pragma unroll
-            for (int k = 0; k < 3; k++)
-                Fb2[k] = -(-c.mass * (Pj + Pi) * aj[k] + c.nu10 * Mi * sig * inv_den * aj[k]);
-            const float sb2 = c.nu10 * Mi * inv_den * bF;
-            const float db2 = -c.nu10 * Mi * sig * inv_den * inv_den * bF;
-#pragma unroll
-            for (int k = 0; k < 3; k++) Fb[k] += w * Fb2[k];
-            sb += w * sb2;
-            db += w * db2;
+            for (int q = 0; q < 3; q++)
+                Fb[q] = (-k.mass * (Pi + Pj) + k.nu10 * Mj * sig * inv_den) * ai[q] -
+                        w * (-k.mass * (Pj + Pi) + k.nu10 * Mi * sig * inv_den) * aj[q];
+            const double sb = k.nu10 * Mj * inv_den * aF + w * k.nu10 * Mi * inv_den * bF;
+            const double db = -k.nu10 * Mj * sig * inv_den * inv_den * aF - w * k.nu10 * Mi * sig * inv_den * inv_den * bF;
             // F = x_ij G / r:  dF/dx_ij^T Fb = (G/r) Fb + (Fb . n)(G' - G/r) n
-            const float Fn = Fb[0] * nh[0] + Fb[1] * nh[1] + Fb[2] * nh[2];
-            const float gr = G * inv_r;
+            const double Fn = Fb[0] * nh[0] + Fb[1] * nh[1] + Fb[2] * nh[2];
+            const double gr = G * inv_r;
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                gx[k] += gr * Fb[k] + Fn * (Gp - gr) * nh[k] + sb * vij[k] + 2.0f * db * xij[k];
-                gv[k] += sb * xij[k];
+            for (int q = 0; q < 3; q++) {
+                gx[q] += gr * Fb[q] + Fn * (Gp - gr) * nh[q] + sb * vij[q] + 2.0 * db * xij[q];
+                gv[q] += sb * xij[q];
             }
         }
         // viscosity: a_i = ... + nu10 * fv, nu10 = 10 nu
-        nu_local = 10.0 * ((double)ai[0] * fv[0] + (double)ai[1] * fv[1] + (double)ai[2] * fv[2]);
+        nu_local = 10.0 * (ai[0] * fv[0] + ai[1] * fv[1] + ai[2] * fv[2]);
         // boundary pressure: a_i = ... - (Pi + pb) rho0 grad psi
-        const float rhoi = t.rho[i];
-        const float4 bnd = t.bnd[i];
-        const float p = eos(rhoi, c);
-        const float pb = rhoi > 0.0f ? p / c.rho0_sq : 0.0f;
-        const double ag = (double)ai[0] * bnd.x + (double)ai[1] * bnd.y + (double)ai[2] * bnd.z;
-        Pbar += -c.rho0 * ag;
-        const double pbbar = -c.rho0 * ag;
-        const float sbnd = -(Pi + pb) * c.rho0;
-        a.gpsi[i] = make_float4(sbnd * ai[0], sbnd * ai[1], sbnd * ai[2], 0.0f);
-        // EOS terms to rho
-        const double dp = eos_dp(rhoi, c);
         const double rd = rhoi;
-        double rbar = rhoi > 0.0f ? pbbar * dp / c.rho0_sq : 0.0;
-        if (rhoi > 0.0f) rbar += Pbar * (dp / (rd * rd) - 2.0 * (double)p / (rd * rd * rd));
-        if (rhoi > 1e-6f) rbar += Mbar * (-(double)c.mass / (rd * rd));
-        a.rbar[i] = (float)rbar;
-        const float4 g0 = a.gx[i], v0 = a.gv[i];
-        a.gx[i] = make_float4((float)(g0.x + gx[0]), (float)(g0.y + gx[1]), (float)(g0.z + gx[2]), 0.0f);
-        a.gv[i] = make_float4((float)(v0.x + gv[0]), (float)(v0.y + gv[1]), (float)(v0.z + gv[2]), 0.0f);
+        const float4 bnd = t.bnd[i];
+        const double p = eos_d(rd, k);
+        const double pb = rhoi > 0.0f ? p / (k.rho0 * k.rho0) : 0.0;
+        const double ag = ai[0] * bnd.x + ai[1] * bnd.y + ai[2] * bnd.z;
+        Pbar += -k.rho0 * ag;
+        const double sbnd = -(Pi + pb) * k.rho0;
+        a.gpsi[i] = make_double4(sbnd * ai[0], sbnd * ai[1], sbnd * ai[2], 0.0);
+        // EOS terms to rho
+        const double dp = eos_dp_d(rd, k);
+        double rbar = rhoi > 0.0f ? -k.rho0 * ag * dp / (k.rho0 * k.rho0) : 0.0;
+        if (rhoi > 0.0f) rbar += Pbar * (dp / (rd * rd) - 2.0 * p / (rd * rd * rd));
+        if (rhoi > 1e-6f) rbar += Mbar * (-k.mass / (rd * rd));
+        a.rbar[i] = rbar;
+        const double4 g0 = a.gx[i], v0 = a.gv[i];
+        a.gx[i] = make_double4(g0.x + gx[0], g0.y + gx[1], g0.z + gx[2], 0.0);
+        a.gv[i] = make_double4(v0.x + gv[0], v0.y + gv[1], v0.z + gv[2], 0.0);
     }
     // block sum of the viscosity adjoint
 #pragma unroll
@@ -393,39 +503,36 @@ __global__ __launch_bounds__(256) void k_add_partials(const double* __restrict__
 
 // Density and boundary volume backwards, then the adjoints of the substep's
 // input in the order of its input arrays (inverse of the reorder).
-__global__ __launch_bounds__(256) void k_adj_density(TapeView t, AdjBuffers a, SimConsts c,
+__global__ __launch_bounds__(256) void k_adj_density(TapeView t, AdjBuffers a, SimConsts c, AdjConsts k,
                                                      const DevObject* __restrict__ objs) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= a.n) return;
     const float4 xi = t.xs[i];
-    const float rbi = a.rbar[i];
+    const double rbi = a.rbar[i];
     const bool dup_i = a.dup[i] != 0u;
-    const float4 g0 = a.gx[i];
+    const double4 g0 = a.gx[i];
     double gx[3] = {g0.x, g0.y, g0.z};
     const uint32_t count = t.cnt[i];
     for (uint32_t e = 0; e < count; e++) {
         const uint32_t j = t.nbr[(size_t)(e >> 2) * 4u * a.n + 4u * i + (e & 3u)];
         if (j == i) continue;
         const float4 xj = t.xs[j];
-        const float xij[3] = {xi.x - xj.x, xi.y - xj.y, xi.z - xj.z};
-        const float r = sqrtf(dot3(xij[0], xij[1], xij[2], xij[0], xij[1], xij[2]));
-        if (!(r > 1e-9f)) continue;
-        const float w = pair_weight(xi, xj, dup_i, a.dup[j] != 0u, c);
-        const double s = (double)(c.mass * (c.h * kernel_dw(r, c))) * ((double)rbi + w * (double)a.rbar[j]) / r;
+        const double xij[3] = {(double)xi.x - xj.x, (double)xi.y - xj.y, (double)xi.z - xj.z};
+        const double r = sqrt(dot3d(xij[0], xij[1], xij[2], xij[0], xij[1], xij[2]));
+        if (!(r > (double)1e-9f)) continue;
+        const double w = pair_weight(xi, xj, dup_i, a.dup[j] != 0u, c);
+        const double s = k.mass * (k.h * dw_d(r, k)) * (rbi + w * a.rbar[j]) / r;
 #pragma unroll
-        for (int k = 0; k < 3; k++) gx[k] += s * xij[k];
+        for (int q = 0; q < 3; q++) gx[q] += s * xij[q];
     }
     if (c.obj_count) {
-        const float4 gp = a.gpsi[i];
-        const float gb[3] = {gp.x, gp.y, gp.z};
+        const double4 gp = a.gpsi[i];
+        const double gb[3] = {gp.x, gp.y, gp.z};
         const float pos[3] = {xi.x, xi.y, xi.z};
-        float gbnd[3] = {0.0f, 0.0f, 0.0f};
-        boundary_volume_adjoint(objs + c.obj_begin, c.obj_count, pos, c, c.rho0 * rbi, gb, gbnd);
-#pragma unroll
-        for (int k = 0; k < 3; k++) gx[k] += gbnd[k];
+        boundary_volume_adjoint(objs + c.obj_begin, c.obj_count, pos, k, k.rho0 * rbi, gb, gx);
     }
     const uint32_t o = t.perm[i];
-    a.gx_out[o] = make_float4((float)gx[0], (float)gx[1], (float)gx[2], 0.0f);
+    a.gx_out[o] = make_double4(gx[0], gx[1], gx[2], 0.0);
     a.gv_out[o] = a.gv[i];
 }
 
@@ -436,6 +543,44 @@ struct GridDesc {
     uint32_t dims[3];
 };
 
+// The loss evaluates the cubic spline in double from the float positions and
+// grid points: it is a small difference of two density fields, which float
+// kernel values would blur by ~1e-3 of its size.  Constants from the float
+// smoothing radius and target density, as the model's FP64 reading.
+struct LossKernel {
+    double h2, inv_h, h, sigma, grad_sigma, mass;
+};
+
+__device__ __forceinline__ double loss_w(double r, const LossKernel& k) {
+    const double q = r * k.inv_h;
+    if (q <= 1.0) {
+        if (q > 0.5) {
+            const double t = 1.0 - q;
+            return (k.sigma * 2.0) * ((t * t) * t);
+        }
+        const double q2 = q * q;
+        const double q3 = q2 * q;
+        return k.sigma * ((6.0 * q3 - 6.0 * q2) + 1.0);
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ double loss_dw(double r, const LossKernel& k) {
+    const double q = r * k.inv_h;
+    if ((r > (double)1e-9f) && (q <= 1.0)) {
+        if (q > 0.5) {
+            const double t = 1.0 - q;
+            return (-k.grad_sigma) * (t * t);
+        }
+        return (k.grad_sigma * q) * (3.0 * q - 2.0);
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ double dot3d(double ax, double ay, double az) {
+    return fma(az, az, fma(ay, ay, ax * ax));
+}
+
 __device__ __forceinline__ void grid_point(const GridDesc& gd, uint32_t idx, float* g) {
     const uint32_t ix = idx % gd.dims[0], iy = (idx / gd.dims[0]) % gd.dims[1], iz = idx / (gd.dims[0] * gd.dims[1]);
     g[0] = gd.o[0] + (float)ix * gd.d[0];
@@ -444,7 +589,7 @@ __device__ __forceinline__ void grid_point(const GridDesc& gd, uint32_t idx, flo
 }
 
 __global__ __launch_bounds__(256) void k_grid_density(GridDesc gd, const float4* __restrict__ xs,
-                                                      const uint32_t* __restrict__ cs, SimConsts c,
+                                                      const uint32_t* __restrict__ cs, SimConsts c, LossKernel lk,
                                                       double* __restrict__ out) {
     const uint32_t idx = blockIdx.x * 256u + threadIdx.x;
     const uint32_t total = gd.dims[0] * gd.dims[1] * gd.dims[2];
@@ -453,7 +598,6 @@ __global__ __launch_bounds__(256) void k_grid_density(GridDesc gd, const float4*
     grid_point(gd, idx, g);
     uint32_t k[27];
     stencil_keys(g[0], g[1], g[2], c, k);
-    // the loss is a small difference of densities: sum in double
     double sum = 0.0;
     for (int u = 0; u < 27; u++) {
         bool seen = false;
@@ -461,9 +605,9 @@ __global__ __launch_bounds__(256) void k_grid_density(GridDesc gd, const float4*
         if (seen) continue;
         for (uint32_t j = cs[k[u]]; j < cs[k[u] + 1u]; j++) {
             const float4 q = xs[j];
-            const float dx = q.x - g[0], dy = q.y - g[1], dz = q.z - g[2];
-            const float r2 = dot3(dx, dy, dz, dx, dy, dz);
-            if (!(r2 > c.h2)) sum += (double)(c.mass * kernel_w(sqrtf(r2), c));
+            const double dx = (double)q.x - g[0], dy = (double)q.y - g[1], dz = (double)q.z - g[2];
+            const double r2 = dot3d(dx, dy, dz);
+            if (!(r2 > lk.h2)) sum += lk.mass * loss_w(sqrt(r2), lk);
         }
     }
     out[idx] = sum;
@@ -474,7 +618,7 @@ __global__ __launch_bounds__(256) void k_grid_density(GridDesc gd, const float4*
 __global__ __launch_bounds__(256) void k_grid_loss_grad(GridDesc gd, const float4* __restrict__ x, uint32_t n,
                                                         const double* __restrict__ rho_g,
                                                         const float* __restrict__ target, SimConsts c,
-                                                        float4* __restrict__ gx) {
+                                                        LossKernel lk, double4* __restrict__ gx) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= n) return;
     const float4 p = x[i];
@@ -495,18 +639,18 @@ __global__ __launch_bounds__(256) void k_grid_loss_grad(GridDesc gd, const float
                 const uint32_t idx = (uint32_t)ix + gd.dims[0] * ((uint32_t)iy + gd.dims[1] * (uint32_t)iz);
                 float g[3];
                 grid_point(gd, idx, g);
-                const float dx = pp[0] - g[0], dy = pp[1] - g[1], dz = pp[2] - g[2];
-                const float r2 = dot3(dx, dy, dz, dx, dy, dz);
-                if (r2 > c.h2) continue;
-                const float r = sqrtf(r2);
-                if (!(r > 1e-9f)) continue;
+                const double dx = (double)pp[0] - g[0], dy = (double)pp[1] - g[1], dz = (double)pp[2] - g[2];
+                const double r2 = dot3d(dx, dy, dz);
+                if (r2 > lk.h2) continue;
+                const double r = sqrt(r2);
+                if (!(r > 1e-9)) continue;
                 const double cg = 2.0 * (rho_g[idx] - (target ? (double)target[idx] : 0.0)) / total;
-                const double s = cg * (double)(c.mass * (c.h * kernel_dw(r, c)) / r);
+                const double s = cg * lk.mass * (lk.h * loss_dw(r, lk)) / r;
                 acc[0] += s * dx;
                 acc[1] += s * dy;
                 acc[2] += s * dz;
             }
-    gx[i] = make_float4((float)acc[0], (float)acc[1], (float)acc[2], 0.0f);
+    gx[i] = make_double4(acc[0], acc[1], acc[2], 0.0);
 }
 
 __global__ void k_max_u32(const uint32_t* __restrict__ a, uint32_t n, uint32_t* __restrict__ out) {
@@ -537,12 +681,56 @@ __global__ void k_d_to_f(const double* __restrict__ src, float* __restrict__ dst
     if (i < n) dst[i] = (float)src[i];
 }
 
-__global__ void k_zero_f4(float4* __restrict__ a, uint32_t n) {
+__global__ void k_zero_d4(double4* __restrict__ a, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < n) a[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+__global__ void k_d4_to_f3(const double4* __restrict__ src, float* __restrict__ dst, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double4 v = src[i];
+        dst[3 * (size_t)i] = (float)v.x;
+        dst[3 * (size_t)i + 1] = (float)v.y;
+        dst[3 * (size_t)i + 2] = (float)v.z;
+    }
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t bs) { return (uint32_t)((n + bs - 1) / bs); }
+
+// The model's constants in double from its float parameters (the FP64
+// reading: 48 / (pi h^4) / h, rho0 * 0.8 h^3 / 8, 2 (d + 2) nu, 0.01 h^2).
+// The reverse pass is checked against the FP64 adjoint of the FP32
+// trajectory, which uses exactly these.
+AdjConsts adj_consts(const SimConsts& c) {
+    AdjConsts k;
+    k.h = (double)c.h;
+    k.inv_h = 1.0 / k.h;
+    k.h2 = k.h * k.h;
+    k.sigma = 8.0 / (M_PI * std::pow(k.h, 3));
+    k.grad_sigma = 48.0 / (M_PI * std::pow(k.h, 4)) / k.h;
+    k.mass = (double)c.rho0 * ((double)0.8f * k.h * k.h * k.h / 8.0);
+    k.dt = (double)c.dt;
+    k.kwall_neg = (double)c.kwall_neg;
+    k.nu10 = 10.0 * (double)c.nu;
+    k.visc_eps_h2 = (double)0.01f * k.h2;
+    k.rho0 = (double)c.rho0;
+    k.B = (double)c.B;
+    k.gamma = (double)c.gamma;
+    k.gamma_int = c.gamma_int;
+    return k;
+}
+
+LossKernel loss_kernel(const SimConsts& c) {
+    LossKernel k;
+    k.h = (double)c.h;
+    k.h2 = k.h * k.h;
+    k.inv_h = 1.0 / k.h;
+    k.sigma = 8.0 / (M_PI * std::pow(k.h, 3));
+    k.grad_sigma = 48.0 / (M_PI * std::pow(k.h, 4)) / k.h;
+    k.mass = (double)c.rho0 * ((double)0.8f * k.h * k.h * k.h / 8.0);
+    return k;
+}
 
 GridDesc to_desc(const float* origin, const float* spacing, const uint32_t* dims) {
     GridDesc g;
@@ -558,21 +746,22 @@ GridDesc to_desc(const float* origin, const float* spacing, const uint32_t* dims
 
 // ---- launchers (declared in adjoint_internal.h)
 int launch_adjoint_substep(const uint32_t* perm, const float4* xs, const float4* vs, const float* rho,
-                           const float4* bnd, const uint32_t* nbr, const uint32_t* cnt, const float4* gx_in,
-                           const float4* gv_in, float4* abar, float4* gx, float4* gv, float* rbar, float4* gpsi,
-                           uint32_t* dup, float4* gx_out, float4* gv_out, double* nu_part, double* nu_bar,
-                           uint32_t n, const SimConsts& c, const DevObject* objs, hipStream_t s) {
+                           const float4* bnd, const uint32_t* nbr, const uint32_t* cnt, const double4* gx_in,
+                           const double4* gv_in, double4* abar, double4* gx, double4* gv, double* rbar,
+                           double4* gpsi, uint32_t* dup, double4* gx_out, double4* gv_out, double* nu_part,
+                           double* nu_bar, uint32_t n, const SimConsts& c, const DevObject* objs, hipStream_t s) {
     if (n == 0) return 0;
     const TapeView t{perm, xs, vs, rho, bnd, nbr, cnt};
     const AdjBuffers a{gx_in, gv_in, abar, gx, gv, rbar, gpsi, dup, gx_out, gv_out, nu_part, n};
     const uint32_t nb = blocks_for(n, 256);
-    hipLaunchKernelGGL(k_adj_integrate, dim3(nb), dim3(256), 0, s, t, a, c);
+    const AdjConsts k = adj_consts(c);
+    hipLaunchKernelGGL(k_adj_integrate, dim3(nb), dim3(256), 0, s, t, a, c, k);
     VFS_TRY(check_launch("adj_integrate"));
-    hipLaunchKernelGGL(k_adj_forces, dim3(nb), dim3(256), 0, s, t, a, c);
+    hipLaunchKernelGGL(k_adj_forces, dim3(nb), dim3(256), 0, s, t, a, c, k);
     VFS_TRY(check_launch("adj_forces"));
     hipLaunchKernelGGL(k_add_partials, dim3(1), dim3(256), 0, s, nu_part, nb, nu_bar);
     VFS_TRY(check_launch("add_partials"));
-    hipLaunchKernelGGL(k_adj_density, dim3(nb), dim3(256), 0, s, t, a, c, objs);
+    hipLaunchKernelGGL(k_adj_density, dim3(nb), dim3(256), 0, s, t, a, c, k, objs);
     return check_launch("adj_density");
 }
 
@@ -581,16 +770,18 @@ int launch_grid_density(const float* origin, const float* spacing, const uint32_
     const GridDesc g = to_desc(origin, spacing, dims);
     const uint64_t total = (uint64_t)dims[0] * dims[1] * dims[2];
     if (total == 0) return 0;
-    hipLaunchKernelGGL(k_grid_density, dim3(blocks_for(total, 256)), dim3(256), 0, s, g, xs, cell_start, c, out);
+    hipLaunchKernelGGL(k_grid_density, dim3(blocks_for(total, 256)), dim3(256), 0, s, g, xs, cell_start, c,
+                       loss_kernel(c), out);
     return check_launch("grid_density");
 }
 
 int launch_grid_loss_grad(const float* origin, const float* spacing, const uint32_t* dims, const float4* x,
-                          uint32_t n, const double* rho_g, const float* target, const SimConsts& c, float4* gx,
+                          uint32_t n, const double* rho_g, const float* target, const SimConsts& c, double4* gx,
                           hipStream_t s) {
     if (n == 0) return 0;
     const GridDesc g = to_desc(origin, spacing, dims);
-    hipLaunchKernelGGL(k_grid_loss_grad, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, x, n, rho_g, target, c, gx);
+    hipLaunchKernelGGL(k_grid_loss_grad, dim3(blocks_for(n, 256)), dim3(256), 0, s, g, x, n, rho_g, target, c,
+                       loss_kernel(c), gx);
     return check_launch("grid_loss_grad");
 }
 
@@ -618,10 +809,16 @@ int launch_d_to_f(const double* src, float* dst, uint32_t n, hipStream_t s) {
     return check_launch("d_to_f");
 }
 
-int launch_zero_f4(float4* a, uint32_t n, hipStream_t s) {
+int launch_zero_d4(double4* a, uint32_t n, hipStream_t s) {
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_zero_f4, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, n);
-    return check_launch("zero_f4");
+    hipLaunchKernelGGL(k_zero_d4, dim3(blocks_for(n, 256)), dim3(256), 0, s, a, n);
+    return check_launch("zero_d4");
+}
+
+int launch_d4_to_f3(const double4* src, float* dst, uint32_t n, hipStream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_d4_to_f3, dim3(blocks_for(n, 256)), dim3(256), 0, s, src, dst, n);
+    return check_launch("d4_to_f3");
 }
 
 }  // namespace vfs

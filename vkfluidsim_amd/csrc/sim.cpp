@@ -59,6 +59,7 @@ SimConsts make_consts(const vfs_params& p) {
                       ? (int32_t)p.expoent
                       : -1;
     c.nu10 = (2.0f * (3.0f + 2.0f)) * p.viscosity_strenght;
+    c.nu = p.viscosity_strenght;
     c.kwall_neg = -p.wall_stiffness;
     c.dt = p.time_scale * p.fixed_dt / (float)p.iterations;
     const float h = c.h;

@@ -99,6 +99,7 @@ struct SimConsts {
     // keys are base + (hash % n).
     uint32_t base;
     uint32_t obj_begin, obj_count;  // its boundary objects in the object array
+    float nu;              // viscosity_strenght (the adjoint's FP64 prefactor 10 nu)
 };
 
 // Constants of the Lague (Clavet double-density) model (lague_model.slang,
