@@ -1,17 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the WCSPH substep engine on one MI355X.
 
-Default workload: 1,048,576-particle WCSPH (128 x 64 x 128 lattice, sphere
-obstacle from a 128^3 SDF/volume map, dt = 1e-4).  A "step" is one substep:
-hash, stable sort, cell ranges, reorder, boundary + density, pressure +
-viscosity + integrate, over every particle.  Inputs are resident in HBM
-before the timed region; the timed region replays the captured substep graph.
+Headline workload: the 1,048,576-particle obstacle scene (128 x 64 x 128
+lattice, icosphere mesh obstacle with 128^3 SDF/volume map, h = 0.012,
+dt = 1e-4) in its contact phase: settled 1,500 substeps from rest so that
+pressure acts on most particles and thousands of keys change bucket per
+substep.  A "step" is one substep: hash, stable sort, cell ranges, reorder,
+boundary + density, pressure + viscosity + integrate, over every particle.
+Inputs are resident in HBM before the timed region; the timed region replays
+the captured substep graph.
 
-Prints one JSON line: particle-substeps/s, the roofline of the dominant kernel
-(compulsory bytes per launch / mean HIP-event duration on the simulation
-stream), its measured HBM traffic per launch (two short child runs of this
-script under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy of known
-size), and the CPU oracle timed on a bounded sample of the same workload.
+Beside it: the same scene from rest as configured (viscosity 0.01, free fall
+until it diverges at the first contact), the 8,388,608-particle sort stress,
+256 batched scenes against per-scene loops, forward + adjoint against
+forward, the Lague dam break, and the CPU oracle on a bounded sample.
+
+Prints one JSON line: particle-substeps/s, the HBM roofline of the dominant
+kernel (algorithmic bytes per launch / mean HIP-event launch time on the
+simulation stream), its measured HBM traffic per launch (child runs of this
+script under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy of
+known size), its VALU issue rate against the peak, and the CPU oracle.
 """
 from __future__ import annotations
 
@@ -29,18 +37,22 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "particle-steps/s @1M WCSPH; batched vs looped scenes/s; adjoint/forward cost"
 HBM_PEAK = 8.0e12
-# Whole-substep compulsory bytes per particle (hash 16, sort 28, ranges 8,
-# reorder 52, density 16, forces + integrate 52): every array read or written
-# once, neighbour re-reads as cache hits, grids L2-resident.
-B_ALG = 172
+# Algorithmic bytes per particle-substep, per phase: every array read or
+# written once, neighbour re-reads as cache hits, grids L2-resident.
+ALG_BYTES = {"hash": 16, "sort": 28, "cell_ranges": 8, "reorder": 52, "density": 16, "forces_integrate": 52}
+B_ALG = sum(ALG_BYTES.values())  # 172
+# the memory-bound phases timed as one class: sort (with the reorder in its
+# merge) and cell ranges; the hash is fused into the force pass
+MEM_CLASS_BYTES = ALG_BYTES["sort"] + ALG_BYTES["cell_ranges"] + ALG_BYTES["reorder"]  # 88
 
 
 def stage_bytes(list_mean: float, reorder_in_sort: bool = False) -> dict:
-    """Compulsory bytes per particle of each stage of this engine, with the
+    """Bytes per particle that this engine's stages must move, with the
     neighbour list (4 bytes per entry, list_mean entries per particle) that the
-    density pass writes and the force pass reads.  A re-sorting substep
-    reorders the state in the sort's merge kernel (reorder_in_sort): the
-    reorder's bytes less the permutation's re-read count to the sort."""
+    density pass writes and the force pass reads (reported beside the
+    algorithmic bytes, never as them).  A re-sorting substep reorders the
+    state in the sort's merge kernel (reorder_in_sort): the reorder's bytes
+    less the permutation's re-read count to the sort."""
     L = 4.0 * list_mean
     return {
         # keys in, sorted keys + permutation out, table fill
@@ -77,7 +89,7 @@ def kernel_base(name: str) -> str:
     return name.split("(")[0].split("<")[0].split("::")[-1].split(" ")[-1]
 
 
-def measure_traffic(workload: str, timeout: float = 240.0) -> dict:
+def measure_traffic(workload: str, state: str | None, timeout: float = 240.0) -> dict:
     """HBM bytes per launch of each stage kernel: two child runs of this script
     under rocprofv3 --pmc (FETCH_SIZE, then WRITE_SIZE), each counter scaled by
     what it reads for a device-to-device copy of CALIBRATION_MIB MiB (which
@@ -98,7 +110,7 @@ def measure_traffic(workload: str, timeout: float = 240.0) -> dict:
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             d = Path(tmp) / counter
             cmd = [prof, "--pmc", counter, "-d", str(d), "-o", "run", "--", sys.executable, str(ROOT / "bench.py"),
-                   "--traffic-probe", "--workload", workload]
+                   "--traffic-probe", "--workload", workload] + (["--state", state] if state else [])
             try:
                 r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout)
             except subprocess.TimeoutExpired:
@@ -137,9 +149,10 @@ def measure_traffic(workload: str, timeout: float = 240.0) -> dict:
 VALU_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU")
 SIMDS = 256 * 4          # MI355X: 256 CUs of 4 SIMD16 units; a wave64 VALU instruction issues over 4 cycles
 PEAK_CLOCK_GHZ = 2.4     # MI355X peak engine clock
+VALU_PEAK = SIMDS * PEAK_CLOCK_GHZ * 1e9 / 4.0  # wave64 VALU instructions per second
 
 
-def measure_valu(workload: str, timeout: float = 240.0) -> dict:
+def measure_valu(workload: str, state: str | None, timeout: float = 240.0) -> dict:
     """Per launch of the density and force kernels: VALU instructions per wave,
     the fraction of lanes active in them, and the VALU instructions as cycles
     per SIMD (4 per wave64 instruction) for bench-side timing to compare with
@@ -156,7 +169,7 @@ def measure_valu(workload: str, timeout: float = 240.0) -> dict:
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
         d = Path(tmp) / "valu"
         cmd = [prof, "--pmc", *VALU_COUNTERS, "-d", str(d), "-o", "run", "--", sys.executable,
-               str(ROOT / "bench.py"), "--traffic-probe", "--workload", workload]
+               str(ROOT / "bench.py"), "--traffic-probe", "--workload", workload] + (["--state", state] if state else [])
         try:
             subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout)
         except subprocess.TimeoutExpired:
@@ -177,6 +190,7 @@ def measure_valu(workload: str, timeout: float = 240.0) -> dict:
             continue
         mean = {c: sum(v) / len(v) for c, v in per.items()}
         out[stage] = {
+            "valu_insts": mean["SQ_INSTS_VALU"],
             "valu_insts_per_wave": mean["SQ_INSTS_VALU"] / mean["SQ_WAVES"],
             "lane_utilization": mean["SQ_THREAD_CYCLES_VALU"] / (64.0 * mean["SQ_ACTIVE_INST_VALU"]),
             "valu_cycles_per_simd": 4.0 * mean["SQ_INSTS_VALU"] / SIMDS,
@@ -206,8 +220,42 @@ def candidates_per_particle(x: np.ndarray, cell: float, n: int) -> float:
     return total / max(1, len(x))
 
 
-def traffic_probe(workload: str) -> int:
-    """Child of measure_traffic: one calibration copy, then two substeps."""
+def workload_params(V, w, flags: int = 0):
+    return V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos, flags=flags, **w.params_kw)
+
+
+def settled_state(w, grids, path: str | None = None):
+    """(x, v) of workload w after its settling substeps: read from `path` (an
+    npz that a --make-state child of this run wrote) or computed here."""
+    import vkfluidsim_amd as V
+
+    if path and Path(path).exists():
+        d = np.load(path)
+        return d["x"], d["v"]
+    if w.settle == 0:
+        return w.x, w.v
+    sim = V.Sim(workload_params(V, w), grids)
+    sim.set_state(w.x, w.v)
+    sim.substeps(w.settle)
+    x, v, _, _ = sim.get_state()
+    sim.close()
+    return x, v
+
+
+def make_state(workload: str, path: str) -> int:
+    """Child: settle the workload and write its state for the probes and the parent."""
+    import vkfluidsim_amd as V
+    from vkfluidsim_amd import configs as CF
+
+    w = CF.WORKLOADS[workload]()
+    x, v = settled_state(w, V.boundary_grids(w.shapes, w.h))
+    np.savez(path, x=x, v=v)
+    return 0
+
+
+def traffic_probe(workload: str, state: str | None) -> int:
+    """Child of measure_traffic / measure_valu: one calibration copy, then
+    three substeps from the workload's (settled) state."""
     import vkfluidsim_amd as V
     from vkfluidsim_amd import configs as CF
 
@@ -218,32 +266,33 @@ def traffic_probe(workload: str) -> int:
     dst.free()
     w = CF.WORKLOADS[workload]()
     grids = V.boundary_grids(w.shapes, w.h)
-    sim = V.Sim(V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos), grids)
-    sim.set_state(w.x, w.v)
+    x, v = settled_state(w, grids, state)
+    sim = V.Sim(workload_params(V, w), grids)
+    sim.set_state(x, v)
     sim.substeps(3)
     sim.synchronize()
     sim.close()
     return 0
 
 
-def cpu_baseline(w, grids, steps: int) -> dict:
-    """The FP32 CPU oracle (OpenMP) on the same workload from its initial state."""
+def cpu_baseline(w, grids, x, v, steps: int) -> dict:
+    """The FP32 CPU oracle (OpenMP) on the same workload from the same state."""
     sys.path.insert(0, str(ROOT / "tests"))
     import harness as H
     import vkfluidsim_amd as V
 
     threads = H.threads()
-    p = V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos)
+    p = workload_params(V, w)
     o = H.OracleSim(p, grids, nthreads=threads)
-    o.set_state(w.x, w.v)
+    o.set_state(x, v)
     o.substeps(1)  # first touch / page-in
     t0 = time.perf_counter()
     o.substeps(steps)
     dt = time.perf_counter() - t0
     del o
     return {"value": w.n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} substeps of the {w.n}-particle workload after its first substep "
-                      f"(FP32 oracle, OpenMP, {dt:.1f} s)"}
+            "sample": f"{steps} substeps of the {w.n}-particle workload ({w.name}, from the timed state) after "
+                      f"its first substep (FP32 oracle, OpenMP, {dt:.1f} s)"}
 
 
 def bench_batched(n_scenes: int, steps: int, looped_steps: int) -> dict:
@@ -278,6 +327,16 @@ def bench_batched(n_scenes: int, steps: int, looped_steps: int) -> dict:
     list_mean, list_max, list_over = batch.neighbour_stats()
     stages = batch.profile(3)
     batch.close()
+    out = {
+        "workload": f"{n_scenes} dam breaks, {total} particles (log-uniform 2048-65536 per scene), 32^3 wall maps",
+        "batched": {"particle_steps_per_s": total / tb, "scene_steps_per_s": n_scenes / tb,
+                    "ms_per_substep": tb * 1e3, "substeps": steps,
+                    "tuning": list(tuning.as_tuple()), "tuning_keeps_reference_hash": tuning.cell_scale == 1.0,
+                    "stages_ms": {name: round(ms, 4) for name, ms, _ in stages}, "finite": finite,
+                    "neighbour_list": {"mean": round(list_mean, 3), "max": list_max, "overflowed": list_over}},
+    }
+    if looped_steps == 0:
+        return out
 
     sims = []
     for k, w in enumerate(ws):
@@ -310,13 +369,7 @@ def bench_batched(n_scenes: int, steps: int, looped_steps: int) -> dict:
     V._lib.call("vfs_stream_destroy", stream)
     for sim in sims:
         sim.close()
-    return {
-        "workload": f"{n_scenes} dam breaks, {total} particles (log-uniform 2048-65536 per scene), 32^3 wall maps",
-        "batched": {"particle_steps_per_s": total / tb, "scene_steps_per_s": n_scenes / tb,
-                    "ms_per_substep": tb * 1e3, "substeps": steps,
-                    "tuning": list(tuning.as_tuple()), "tuning_keeps_reference_hash": tuning.cell_scale == 1.0,
-                    "stages_ms": {name: round(ms, 4) for name, ms, _ in stages}, "finite": finite,
-                    "neighbour_list": {"mean": round(list_mean, 3), "max": list_max, "overflowed": list_over}},
+    out.update({
         "looped": {"particle_steps_per_s": total / ts, "scene_steps_per_s": n_scenes / ts,
                    "ms_per_substep": ts * 1e3, "substeps": looped_steps,
                    "how": "one vfs_sim per scene, each replaying its substep graph, scene after scene on one stream"},
@@ -326,7 +379,8 @@ def bench_batched(n_scenes: int, steps: int, looped_steps: int) -> dict:
                                       "may overlap on the GPU)"},
         "speedup": ts / tb,
         "speedup_vs_own_streams": tl / tb,
-    }
+    })
+    return out
 
 
 def bench_adjoint(steps: int, every: int, repeats: int = 3) -> dict:
@@ -480,26 +534,128 @@ def summarize_pmc(fetch_db: str, write_db: str) -> str:
     return "\n".join(lines)
 
 
+def time_substeps(sim, warmup: int, steps: int):
+    """Seconds of `steps` graph-replayed substeps after `warmup` untimed ones,
+    bracketed by stream synchronisation."""
+    sim.substeps(warmup)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.substeps(steps)
+    sim.synchronize()
+    return time.perf_counter() - t0
+
+
+def state_summary(sim, w) -> dict:
+    """Sanity of a state and the work it gives the substep: finite, in the box,
+    densities, pressure share, boundary share, list lengths."""
+    list_mean, list_max, list_over = sim.neighbour_stats()
+    x, v, rho, _ = sim.get_state()
+    psi, _ = sim.get_boundary()
+    lo = np.asarray(w.box_pos, np.float32)
+    hi = lo + np.asarray(w.box_size, np.float32)
+    speed = np.linalg.norm(v, axis=1) if w.n else np.zeros(1)
+    return {"finite": bool(np.isfinite(x).all() and np.isfinite(v).all() and np.isfinite(rho).all()),
+            "max_speed": float(speed.max()), "speed_p99": float(np.percentile(speed, 99)),
+            "outside_box": int(np.any((x < lo) | (x > hi), axis=1).sum()),
+            "rho_mean": float(rho.mean()) if w.n else 0.0,
+            "above_rest_density": round(float(np.mean(rho > w.params_kw.get("rho0", 1000.0))), 3),
+            "near_boundary_object": round(float(np.mean(psi > 0)), 4),
+            "neighbour_list": {"mean": round(list_mean, 3), "max": list_max, "overflowed": list_over}}
+
+
+def changed_keys_per_substep(sim, k: int = 5) -> float:
+    """Mean keys that changed bucket over k more substeps (each read after its substep)."""
+    tot = 0
+    for _ in range(k):
+        sim.substeps(1)
+        tot += sim.changed_keys()
+    return tot / k
+
+
+def stage_table(stages):
+    stage_ms = {name: ms for name, ms, _ in stages}
+    launches = {name: la for name, _, la in stages}
+    sort_ms = sum(ms for name, ms in stage_ms.items() if name.startswith("sort"))
+    return stage_ms, launches, sort_ms
+
+
+def mem_class_frac(n: int, stage_ms: dict, sort_ms: float):
+    """The memory-bound class (sort with the reorder in its merge, cell ranges)
+    against its algorithmic bytes (sort 28 + ranges 8 + reorder 52 per particle)."""
+    ms = sort_ms + stage_ms.get("cell_ranges", 0.0) + stage_ms.get("reorder", 0.0)
+    return (MEM_CLASS_BYTES * n / (ms * 1e-3)) / HBM_PEAK if ms > 0 else None
+
+
+def bench_free_fall(steps: int) -> dict:
+    """The 1M obstacle scene as configured (viscosity 0.01) from rest: the lattice
+    falls freely (no pressure, no relative motion) until it reaches the floor at
+    ~320 substeps, where the viscous term diverges; timed over substeps 20-220."""
+    import vkfluidsim_amd as V
+    from vkfluidsim_amd import configs as CF
+
+    w = CF.obstacle_1m()
+    grids = V.boundary_grids(w.shapes, w.h)
+    sim = V.Sim(workload_params(V, w), grids)
+    sim.set_state(w.x, w.v)
+    el = time_substeps(sim, 20, steps)
+    ck = changed_keys_per_substep(sim)
+    out = {"workload": f"{w.name}: {w.n} particles from rest, viscosity 0.01, substeps 20-{20 + steps}",
+           "particle_steps_per_s": w.n * steps / el, "ms_per_substep": 1e3 * el / steps,
+           "changed_keys_per_substep": ck, "state": state_summary(sim, w)}
+    sim.close()
+    return out
+
+
+def bench_sort_stress(warmup: int, steps: int) -> dict:
+    """8,388,608 particles uniform in 4 x 2 x 2 with N(0, 0.5) velocities,
+    h = 0.02, 256^3 wall map: every substep re-sorts a few hundred thousand
+    changed keys through the multi-tile radix passes.  Timed over its first
+    substeps (the random placement ejects particles after ~20 substeps)."""
+    import vkfluidsim_amd as V
+    from vkfluidsim_amd import configs as CF
+
+    w = CF.uniform_stress()
+    grids = V.boundary_grids(w.shapes, w.h)
+    sim = V.Sim(workload_params(V, w), grids)
+    sim.set_state(w.x, w.v)
+    el = time_substeps(sim, warmup, steps)
+    state = state_summary(sim, w)
+    ck = changed_keys_per_substep(sim, 2)
+    stage_ms, launches, sort_ms = stage_table(sim.profile(3))
+    sim.close()
+    return {"workload": f"{w.name}: {w.n} particles, substeps {warmup}-{warmup + steps} from the seeded state",
+            "particle_steps_per_s": w.n * steps / el, "ms_per_substep": 1e3 * el / steps,
+            "changed_keys_per_substep": ck,
+            "stages_ms": {k: round(v, 5) for k, v in stage_ms.items()}, "stage_launches": launches,
+            "memory_class_hbm_frac": mem_class_frac(w.n, stage_ms, sort_ms),
+            "whole_step_hbm_frac": B_ALG * w.n * steps / el / HBM_PEAK, "state": state}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="obstacle_1m")
+    ap.add_argument("--workload", default="obstacle_1m_contact")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 --pmc child runs")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 --pmc child runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-scene batched vs looped measurement")
+    ap.add_argument("--no-looped", action="store_true", help="time the batch alone, without the per-scene loops")
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=200)
     ap.add_argument("--looped-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
     ap.add_argument("--no-lague", action="store_true", help="skip the Lague model measurement")
+    ap.add_argument("--no-extra-legs", action="store_true", help="skip the free-fall and sort-stress legs")
     ap.add_argument("--lague-steps", type=int, default=300, help="timed substeps of the Lague dam break")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20)
+    ap.add_argument("--stress-steps", type=int, default=10)
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--make-state", metavar="NPZ", help=argparse.SUPPRESS)
+    ap.add_argument("--state", metavar="NPZ", help=argparse.SUPPRESS)
     ap.add_argument("--summarize-stats", metavar="DB", help="print a rocprofv3 --stats database as text")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="print the HBM traffic of two --pmc databases of --traffic-probe runs")
@@ -516,14 +672,27 @@ def main() -> int:
               "single-GPU kernel path; multi-GPU sharding is a design note, not a measured path.",
               file=sys.stderr)
         return 2
+    if args.make_state:
+        return make_state(args.workload, args.make_state)
     if args.traffic_probe:
-        return traffic_probe(args.workload)
+        return traffic_probe(args.workload, args.state)
 
-    traffic = {} if args.no_traffic else measure_traffic(args.workload)
-    valu = {} if args.no_traffic else measure_valu(args.workload)
+    import subprocess
+    import tempfile
+
+    from vkfluidsim_amd import configs as CF
+
+    tmpdir = tempfile.TemporaryDirectory(dir="/tmp")
+    state_path = None
+    if CF.WORKLOADS[args.workload]().settle:
+        # the settled state, made once by a child before this process touches the GPU
+        state_path = str(Path(tmpdir.name) / "state.npz")
+        subprocess.run([sys.executable, str(ROOT / "bench.py"), "--workload", args.workload, "--make-state",
+                        state_path], check=True, timeout=600)
+    traffic = {} if args.no_traffic else measure_traffic(args.workload, state_path)
+    valu = {} if args.no_traffic else measure_valu(args.workload, state_path)
 
     import vkfluidsim_amd as V
-    from vkfluidsim_amd import configs as CF
 
     if V.device_count() < 1:
         print("bench.py needs a GPU", file=sys.stderr)
@@ -532,45 +701,41 @@ def main() -> int:
     t_grid = time.perf_counter()
     grids = V.boundary_grids(w.shapes, w.h)
     grid_ms = 1e3 * (time.perf_counter() - t_grid)
-    params = V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos)
-    sim = V.Sim(params, grids)
-    sim.set_state(w.x, w.v)
-
-    sim.substeps(args.warmup)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    sim.substeps(args.steps)
-    sim.synchronize()
-    elapsed = time.perf_counter() - t0
+    x0, v0 = settled_state(w, grids, state_path)
+    sim = V.Sim(workload_params(V, w), grids)
+    sim.set_state(x0, v0)
+    elapsed = time_substeps(sim, args.warmup, args.steps)
     ms_per_step = 1e3 * elapsed / args.steps
     value = w.n * args.steps / elapsed
 
-    list_mean, list_max, list_over = sim.neighbour_stats()
-    x, v, rho, _ = sim.get_state()
-    finite = bool(np.isfinite(x).all() and np.isfinite(v).all() and np.isfinite(rho).all())
-    lo = np.asarray(w.box_pos, np.float32)
-    hi = lo + np.asarray(w.box_size, np.float32)
-    end_state = {"substeps_run": args.warmup + args.steps, "finite": finite,
-                 "max_speed": float(np.linalg.norm(v, axis=1).max()) if w.n else 0.0,
-                 "outside_box": int(np.any((x < lo) | (x > hi), axis=1).sum()),
-                 "rho_mean": float(rho.mean()) if w.n else 0.0,
-                 "neighbour_list": {"mean": round(list_mean, 3), "max": list_max, "overflowed": list_over}}
-    cand = candidates_per_particle(x, w.h, w.n) if valu and w.n else None
-    stages = sim.profile(args.profile_steps)
+    end_state = state_summary(sim, w)
+    end_state["substeps_run"] = w.settle + args.warmup + args.steps
+    list_mean = end_state["neighbour_list"]["mean"]
+    x_end, _, _, _ = sim.get_state()
+    cand = candidates_per_particle(x_end, w.h, w.n) if valu and w.n else None
+    changed = changed_keys_per_substep(sim)
+    stage_ms, launches, sort_ms = stage_table(sim.profile(args.profile_steps))
     sim.close()
 
-    stage_ms = {name: ms for name, ms, _ in stages}
-    launches = {name: la for name, _, la in stages}
-    sort_ms = sum(ms for name, ms in stage_ms.items() if name.startswith("sort"))
     per = dict((k, v) for k, v in stage_ms.items() if not k.startswith("sort"))
     per["sort"] = sort_ms
     dom = max(per, key=per.get)
     dom_launch_ms = per[dom] / max(1, launches.get(dom, 1)) if dom != "sort" else sort_ms
+    alg = ALG_BYTES.get(dom, 0) + (ALG_BYTES["reorder"] if dom == "sort" else 0)
+    achieved = alg * w.n / (dom_launch_ms * 1e-3)
     sb = stage_bytes(list_mean, reorder_in_sort="reorder" not in per)
-    achieved = sb[dom] * w.n / (dom_launch_ms * 1e-3)
-    mem_class = [k for k in ("sort", "cell_ranges", "reorder") if k in per]
-    mem_ms = sum(per[k] for k in mem_class)
-    mem_bytes = sum(sb[k] for k in mem_class) * w.n
+    dom_valu = valu.get(dom)
+
+    def valu_obj(k, v):
+        t = per[k] / max(1, launches.get(k, 1)) * 1e-3
+        item = (cand or 1.0) if k == "density" else max(list_mean, 1.0)
+        return {"valu_insts_per_wave": round(v["valu_insts_per_wave"]),
+                "work_per_particle": ({"candidates": round(cand, 1)} if k == "density" and cand
+                                      else {"listed_neighbours": round(list_mean, 1)}),
+                "valu_insts_per_item": round(v["valu_insts_per_wave"] / item, 1),
+                "lane_utilization": round(v["lane_utilization"], 3),
+                "valu_busy_frac": round(v["valu_cycles_per_simd"] / (t * PEAK_CLOCK_GHZ * 1e9), 3),
+                "clock_ghz_assumed": PEAK_CLOCK_GHZ}
 
     out = {
         "metric": METRIC,
@@ -586,46 +751,49 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": w.name, "n_particles": w.n, "smooth_radius": w.h, "dt": w.dt,
+                   "viscosity": w.params_kw.get("viscosity", 0.01), "settling_substeps": w.settle,
                    "boundary_grids": [list(g.resolution) for g in grids],
                    "boundary_objects": [describe_shape(sh) for sh in w.shapes], "substep_graph": True},
         "setup": {"boundary_grids_ms": round(grid_ms, 1),
                   "what": "SDF + volume map of every boundary object generated on the GPU (FP64, 16^3 "
                           "Gauss-Legendre per node), host copies included"},
+        # HBM roofline of the dominant kernel against its algorithmic bytes
+        # (density: read x 12 + write rho 4 per particle); the bytes this
+        # engine must move (its neighbour lists included) and the measured
+        # traffic beside them
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic.get(dom) if dom in traffic else None,
-                     "algorithmic_bytes": sb[dom] * w.n, "bytes_per_particle": round(sb[dom], 2),
-                     "launch_ms": dom_launch_ms},
-        # the neighbour kernels are bound by VALU issue, not HBM: VALU busy =
-        # VALU cycles per SIMD / (launch time x peak clock); per work item:
-        # density per candidate tested, forces per listed neighbour
-        "valu_issue": {k: {"valu_insts_per_wave": round(v["valu_insts_per_wave"]),
-                           "work_per_particle": ({"candidates": round(cand, 1)} if k == "density"
-                                                 else {"listed_neighbours": round(list_mean, 1)}),
-                           "valu_insts_per_item": round(v["valu_insts_per_wave"]
-                                                        / ((cand or 1.0) if k == "density" else max(list_mean, 1.0)), 1),
-                           "lane_utilization": round(v["lane_utilization"], 3),
-                           "valu_busy_frac": round(v["valu_cycles_per_simd"]
-                                                   / (per[k] / max(1, launches.get(k, 1)) * 1e-3
-                                                      * PEAK_CLOCK_GHZ * 1e9), 3),
-                           "clock_ghz_assumed": PEAK_CLOCK_GHZ}
-                       for k, v in valu.items() if k in per} or None,
+                     "algorithmic_bytes": alg * w.n, "bytes_per_particle": alg, "launch_ms": dom_launch_ms,
+                     "engine_bytes_per_particle": round(sb.get(dom, 0.0), 2),
+                     "engine_bytes_frac": sb.get(dom, 0.0) * w.n / (dom_launch_ms * 1e-3) / HBM_PEAK},
+        # the neighbour kernels are bound by VALU issue: wave64 VALU
+        # instructions per second against the peak issue rate (4 cycles per
+        # instruction on each of the 1,024 SIMDs at the peak clock)
+        "valu_roofline": ({"bound": "valu", "kernel": dom,
+                           "achieved": dom_valu["valu_insts"] / (dom_launch_ms * 1e-3), "peak": VALU_PEAK,
+                           "unit": "wave64 VALU instructions/s",
+                           "frac": dom_valu["valu_insts"] / (dom_launch_ms * 1e-3) / VALU_PEAK}
+                          if dom_valu else None),
+        "valu_issue": {k: valu_obj(k, v) for k, v in valu.items() if k in per} or None,
         "whole_step_hbm_frac": B_ALG * value / HBM_PEAK,
-        "memory_class_hbm_frac": (mem_bytes / (mem_ms * 1e-3)) / HBM_PEAK if mem_ms > 0 else None,
+        "memory_class_hbm_frac": mem_class_frac(w.n, stage_ms, sort_ms),
+        "changed_keys_per_substep": changed,
         "traffic_per_launch": {k: round(v) for k, v in traffic.items() if k in STAGE_KERNELS} or traffic or None,
         "stages_ms": {k: round(v, 5) for k, v in stage_ms.items()},
         "stage_launches": launches,
         "event_sum_ms_per_step": sum(stage_ms.values()),
         "end_state": end_state,
     }
+    if not args.no_extra_legs:
+        out["free_fall"] = bench_free_fall(200)
+        out["sort_stress"] = bench_sort_stress(1, args.stress_steps)
     out["batched"] = None if args.no_batched else bench_batched(args.batched_scenes, args.batched_steps,
-                                                                  args.looped_steps)
+                                                                  0 if args.no_looped else args.looped_steps)
     out["adjoint"] = None if args.no_adjoint else bench_adjoint(args.adjoint_steps, args.adjoint_every)
     out["lague"] = None if args.no_lague else bench_lague(args.lague_steps, 100, not args.no_cpu_baseline)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, grids, args.cpu_steps)
-    else:
-        out["cpu_baseline"] = None
+    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(w, grids, x0, v0, args.cpu_steps)
+    tmpdir.cleanup()
     print(json.dumps(out))
     return 0
 

@@ -197,6 +197,10 @@ int vfs_sim_profile_stages(vfs_sim *sim, uint32_t n, float *ms, uint32_t *launch
  * capacity of VFS_NEIGHBOUR_CAPACITY entries (their forces walk the cells).
  * No counterpart in the reference (it has no neighbour lists). */
 int vfs_sim_neighbour_stats(vfs_sim *sim, double *mean, uint32_t *max_count, uint32_t *overflowed);
+/* Keys that changed bucket in the last substep: the keys its re-sort sorted
+ * and merged (n when that substep sorted every key).  No counterpart in the
+ * reference (it sorts every key every substep, sort.cpp:107-143). */
+int vfs_sim_sort_stats(vfs_sim *sim, uint32_t *changed_keys);
 
 /* ---- batches of independent scenes ------------------------------------ */
 /* No counterpart in the reference (one scene per Simulation, src/simulation.h:49).

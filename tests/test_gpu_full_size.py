@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _params(w, flags=V.VFS_FLAG_TRACK_IDS):
-    return V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos, flags=flags)
+    return V.make_params(w.n, w.h, dt=w.dt, box_size=w.box_size, box_pos=w.box_pos, flags=flags, **w.params_kw)
 
 
 def _same(sim, orc):
@@ -66,6 +66,34 @@ def test_obstacle_1m_substeps_bit_exact():
     del orc
     sim.substeps(10)
     _properties(sim, w.n)
+
+
+def test_obstacle_1m_contact_phase_bit_exact():
+    """The benchmark's contact-phase state: 1,500 substeps on the GPU bring the
+    fluid onto the floor (pressure active on most particles, thousands of keys
+    changing bucket per substep); from that state 2 + 8 substeps are compared
+    with the oracle bit for bit."""
+    w = CF.obstacle_1m_contact()
+    print(f"case obstacle_1m_contact: {w.n} particles, {w.settle} settling substeps", flush=True)
+    grids = V.boundary_grids(w.shapes, w.h)
+    p = _params(w)
+    sim = V.Sim(p, grids)
+    sim.set_state(w.x, w.v)
+    sim.substeps(w.settle)
+    x, v, rho, _ = sim.get_state()
+    assert np.mean(rho > 1000.0) > 0.5  # pressure acts on most particles
+    sim.set_state(x, v)
+    orc = H.OracleSim(p, grids)
+    orc.set_state(x, v)
+    sim.substeps(2)
+    orc.substeps(2)
+    _same(sim, orc)
+    sim.substeps(8)
+    orc.substeps(8)
+    _same(sim, orc)
+    assert sim.changed_keys() > 1000
+    for a, b in zip(sim.get_boundary(), orc.get_boundary()):
+        np.testing.assert_array_equal(np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel())
 
 
 def test_uniform_8m_substep_bit_exact():

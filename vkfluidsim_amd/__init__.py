@@ -402,6 +402,12 @@ class Sim:
         call("vfs_sim_neighbour_stats", self.handle, C.byref(mean), C.byref(mx), C.byref(over))
         return mean.value, mx.value, over.value
 
+    def changed_keys(self) -> int:
+        """Keys that changed bucket in the last substep (n after a full sort)."""
+        c = C.c_uint32(0)
+        call("vfs_sim_sort_stats", self.handle, C.byref(c))
+        return int(c.value)
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             lib.vfs_sim_destroy(self.handle)

@@ -163,6 +163,7 @@ SIGNATURES = {
     "vfs_sim_autotune": (_i32, [_vp, _u32, C.POINTER(Tuning), C.POINTER(C.c_float)]),
     "vfs_sim_profile_stages": (_i32, [_vp, _u32, _fp, _up, C.POINTER(C.c_char_p), _u32, _up]),
     "vfs_sim_neighbour_stats": (_i32, [_vp, C.POINTER(C.c_double), _up, _up]),
+    "vfs_sim_sort_stats": (_i32, [_vp, _up]),
     "vfs_batch_create": (_i32, [_vp, _u32, _vp, _up, C.POINTER(_vp)]),
     "vfs_batch_destroy": (_i32, [_vp]),
     "vfs_batch_info": (_i32, [_vp, _up, C.POINTER(C.c_uint64), _up]),

@@ -71,6 +71,8 @@ class Workload:
     # analytic boundary shapes: (kind, params, resolution)
     shapes: list = field(default_factory=list)
     params_kw: dict = field(default_factory=dict)
+    # substeps run from (x, v) on the GPU before the workload's measured state
+    settle: int = 0
 
 
 def icosphere(subdivisions: int, radius: float = 1.0, center=(0.0, 0.0, 0.0)):
@@ -192,6 +194,24 @@ def obstacle_1m(spacing: float = 0.006, map_res: int = 128, side=(128, 64, 128),
                     shapes)
 
 
+def obstacle_1m_contact(settle: int = 1500) -> Workload:
+    """The 1,048,576-particle obstacle workload in its contact phase: the same
+    lattice, box, obstacle, h = 0.012 and dt = 1e-4, with viscosity 1e-4, run
+    `settle` substeps from rest so that the fluid has hit the floor (at 1,500
+    substeps two thirds of the particles are above rest density, ~6,000 keys
+    change bucket per substep, 36 listed neighbours per particle).
+
+    Viscosity 0.01 is unstable here at any practical dt: with the reference's
+    kernel gradient (an extra 1/h) the viscous term's rate grows like
+    nu / h^3, about 1.6e6 /s at h = 0.012, so the explicit update diverges at
+    the first contact for dt = 1e-4 and dt = 1e-5 alike (and for nu = 1e-3 at
+    dt = 1e-4); nu = 1e-4 is stable through 3,000 substeps.  The viscous sum
+    is evaluated in every substep whatever nu is."""
+    w = obstacle_1m()
+    return Workload("wcsph_obstacle_1m_contact", w.n, w.h, w.dt, w.steps, w.box_size, w.box_pos, w.x, w.v,
+                    w.shapes, {"viscosity": 1e-4}, settle)
+
+
 def _cbrtf(x: np.float32) -> np.float32:
     """std::cbrt of a float as the reference's host code computes it (the C
     library's cbrtf; numpy's float32 cbrt can differ in the last bit)."""
@@ -233,13 +253,18 @@ def lague_dam_break(counts=(50, 60, 50), steps: int = 100) -> Workload:
 
 
 def uniform_stress(n: int = 8_388_608, h: float = 0.02, map_res: int = 256, steps: int = 200) -> Workload:
-    """n particles uniform in a 4 x 2 x 2 box with N(0, 0.5) velocities, box walls."""
+    """n particles uniform in a 4 x 2 x 2 box with N(0, 0.5) velocities, box
+    walls, dt = 1e-4, viscosity 1e-4.  Random placement puts pairs far closer
+    than the lattice spacing; their pressure, with the reference's 1/h-scaled
+    gradient, ejects particles within ~20 substeps (at any viscosity), so the
+    state is a sort and neighbour stress for its first substeps only."""
     seed = SEED_BASE + 2
     u = uniform(seed, 3 * n).reshape(n, 3)
     x = (u * np.array([4.0, 2.0, 2.0], dtype=np.float32)).astype(np.float32)
     v = (0.5 * normal(seed ^ 0xABCDEF, 3 * n)).astype(np.float32).reshape(n, 3)
     walls = (2, (2.0, 1.0, 1.0, 2.0, 1.0, 1.0), (map_res,) * 3)
-    return Workload("wcsph_uniform_stress", n, h, 1e-4, steps, (4.0, 2.0, 2.0), (0.0, 0.0, 0.0), x, v, [walls])
+    return Workload("wcsph_uniform_stress", n, h, 1e-4, steps, (4.0, 2.0, 2.0), (0.0, 0.0, 0.0), x, v, [walls],
+                    {"viscosity": 1e-4})
 
 
 def batched_dam_breaks(n_scenes: int = 256, seed: int = SEED_BASE + 3, map_res: int = 32, steps: int = 200,
@@ -273,6 +298,7 @@ def batched_dam_breaks(n_scenes: int = 256, seed: int = SEED_BASE + 3, map_res: 
 WORKLOADS = {
     "dam_break_32k": lambda: dam_break(),
     "obstacle_1m": lambda: obstacle_1m(),
+    "obstacle_1m_contact": lambda: obstacle_1m_contact(),
     "lague_dam_break": lambda: lague_dam_break(),
     "obstacle_1m_wall_map": lambda: obstacle_1m(walls=True),
     "uniform_8m": lambda: uniform_stress(),

@@ -98,7 +98,7 @@ int vfs_sim_copy_data_buffers(vfs_sim* sim, const vfs_render_buffers* b, void* s
     VFS_CHECK(b->ready, VFS_ERR_INVALID, "render buffers opened from another process are read-only");
     VFS_CHECK(!b->accelerations || sim->accel, VFS_ERR_STATE, "accelerations need VFS_FLAG_STORE_ACCEL");
     hipStream_t s = stream ? stream_of(stream) : sim->stream;
-    if (s != sim->stream) VFS_TRY(vfs_sim::order_after(s, sim->stream));
+    if (s != sim->stream) VFS_TRY(sim->order_after(s, sim->stream));
     if (sim->n) {
         hipLaunchKernelGGL(k_pack_render, dim3((sim->n + 255u) / 256u), dim3(256), 0, s, sim->x, sim->v, sim->rho,
                            b->accelerations ? sim->accel : nullptr, sim->n, b->positions, b->velocities, b->densities,

@@ -139,6 +139,7 @@ void free_sim(vfs_sim* s) {
         if (p) (void)hipFree(p);
     for (float* g : s->grids)
         if (g) (void)hipFree(g);
+    if (s->order_event) (void)hipEventDestroy(s->order_event);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -574,6 +575,7 @@ int profile_impl(vfs_sim* sim, uint32_t n, float* ms, uint32_t* launches, const 
     VFS_TRY(sim->ensure_keys(sim->stream));
     EventTimer timer;
     for (uint32_t k = 0; k < n; k++) {
+        sim->last_full_sort = !sim->resort_ready;
         VFS_TRY(sim->record_substep(sim->stream, &timer));
         sim->resort_ready = true;
     }
@@ -825,6 +827,21 @@ int vfs_sim_profile_stages(vfs_sim* sim, uint32_t n, float* ms, uint32_t* launch
 
 int vfs_sim_neighbour_stats(vfs_sim* sim, double* mean, uint32_t* max_count, uint32_t* overflowed) {
     return neighbour_stats_impl(sim, mean, max_count, overflowed);
+}
+
+int vfs_sim_sort_stats(vfs_sim* sim, uint32_t* changed_keys) {
+    VFS_CHECK(sim && changed_keys, VFS_ERR_INVALID, "null argument");
+    *changed_keys = 0;
+    if (sim->n == 0 || !sim->stepped) return 0;
+    if (sim->last_full_sort) {
+        *changed_keys = sim->n;
+        return 0;
+    }
+    // the re-sort's count of changed keys stays in its state word until the next substep
+    VFS_HIP(hipMemcpyAsync(changed_keys, sim->sort_ws + sim->plan.off_rs_state, 4, hipMemcpyDeviceToHost,
+                           sim->stream));
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    return 0;
 }
 
 /* ---- batches ----------------------------------------------------------- */

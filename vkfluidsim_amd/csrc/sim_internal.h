@@ -58,6 +58,7 @@ struct vfs_sim {
     // sorted_keys holds an ascending key array and cell_start its bucket
     // starts (true after the first substep): later substeps re-sort
     bool resort_ready = false;
+    bool last_full_sort = false;  // the last substep sorted every key (no re-sort)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     vfs_tuning tuning{8, 16, 128, 1, 1.0f};
@@ -242,13 +243,16 @@ struct vfs_sim {
         sn = Snapshot{};
     }
 
-    // Makes `waiter` wait for the work queued so far on `signaller`.
-    static int order_after(hipStream_t waiter, hipStream_t signaller) {
-        hipEvent_t ev;
-        VFS_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        VFS_HIP(hipEventRecord(ev, signaller));
-        VFS_HIP(hipStreamWaitEvent(waiter, ev, 0));
-        VFS_HIP(hipEventDestroy(ev));
+    // Makes `waiter` wait for the work queued so far on `signaller`, through
+    // one event per simulation (created on first use, destroyed with it): a
+    // wait takes the event's latest record at the time of the call, so later
+    // records do not disturb earlier waits, and substep calls on a caller's
+    // stream create no events.
+    hipEvent_t order_event = nullptr;
+    int order_after(hipStream_t waiter, hipStream_t signaller) {
+        if (!order_event) VFS_HIP(hipEventCreateWithFlags(&order_event, hipEventDisableTiming));
+        VFS_HIP(hipEventRecord(order_event, signaller));
+        VFS_HIP(hipStreamWaitEvent(waiter, order_event, 0));
         return 0;
     }
 
@@ -256,10 +260,12 @@ struct vfs_sim {
         if (n == 0 || count == 0) return 0;
         if (s != stream) VFS_TRY(order_after(s, stream));
         VFS_TRY(ensure_keys(s));
+        last_full_sort = false;
         if (!resort_ready) {
             // the first substep sorts from scratch; the graph records the re-sort
             VFS_TRY(record_substep(s, nullptr));
             resort_ready = true;
+            last_full_sort = count == 1;
             count--;
         }
         if (!(flags & VFS_FLAG_NO_GRAPH)) {
