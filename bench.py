@@ -649,13 +649,16 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
     ap.add_argument("--no-lague", action="store_true", help="skip the Lague model measurement")
     ap.add_argument("--no-extra-legs", action="store_true", help="skip the free-fall and sort-stress legs")
+    ap.add_argument("--no-free-fall", action="store_true", help="skip the free-fall leg")
+    ap.add_argument("--no-sort-stress", action="store_true", help="skip the 8M sort-stress leg")
     ap.add_argument("--lague-steps", type=int, default=300, help="timed substeps of the Lague dam break")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--stress-steps", type=int, default=10)
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--make-state", metavar="NPZ", help=argparse.SUPPRESS)
-    ap.add_argument("--state", metavar="NPZ", help=argparse.SUPPRESS)
+    ap.add_argument("--make-state", metavar="NPZ", help="write the workload's settled state and exit")
+    ap.add_argument("--state", metavar="NPZ", help="settled state of the workload written by --make-state "
+                                                   "(default: made by a child of this run)")
     ap.add_argument("--summarize-stats", metavar="DB", help="print a rocprofv3 --stats database as text")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="print the HBM traffic of two --pmc databases of --traffic-probe runs")
@@ -683,8 +686,8 @@ def main() -> int:
     from vkfluidsim_amd import configs as CF
 
     tmpdir = tempfile.TemporaryDirectory(dir="/tmp")
-    state_path = None
-    if CF.WORKLOADS[args.workload]().settle:
+    state_path = args.state
+    if state_path is None and CF.WORKLOADS[args.workload]().settle:
         # the settled state, made once by a child before this process touches the GPU
         state_path = str(Path(tmpdir.name) / "state.npz")
         subprocess.run([sys.executable, str(ROOT / "bench.py"), "--workload", args.workload, "--make-state",
@@ -785,9 +788,9 @@ def main() -> int:
         "event_sum_ms_per_step": sum(stage_ms.values()),
         "end_state": end_state,
     }
-    if not args.no_extra_legs:
-        out["free_fall"] = bench_free_fall(200)
-        out["sort_stress"] = bench_sort_stress(1, args.stress_steps)
+    out["free_fall"] = None if args.no_extra_legs or args.no_free_fall else bench_free_fall(200)
+    out["sort_stress"] = (None if args.no_extra_legs or args.no_sort_stress
+                          else bench_sort_stress(1, args.stress_steps))
     out["batched"] = None if args.no_batched else bench_batched(args.batched_scenes, args.batched_steps,
                                                                   0 if args.no_looped else args.looped_steps)
     out["adjoint"] = None if args.no_adjoint else bench_adjoint(args.adjoint_steps, args.adjoint_every)

@@ -29,13 +29,20 @@ typedef enum vfs_status {
     VFS_ERR_HIP = -2,       /* a HIP runtime call failed */
     VFS_ERR_STATE = -3,     /* object not in a state that allows the call */
     VFS_ERR_IO = -4,        /* file could not be read / parsed */
-    VFS_ERR_NOMEM = -5      /* allocation failed */
+    VFS_ERR_NOMEM = -5,     /* allocation failed */
+    VFS_ERR_DIVERGED = -6   /* VFS_FLAG_CHECK_STATE: the state holds non-finite or escaped particles */
 } vfs_status;
 
 /* Flags for vfs_params.flags. */
 #define VFS_FLAG_TRACK_IDS 0x1u   /* carry a particle-id buffer through every reorder */
 #define VFS_FLAG_NO_GRAPH 0x2u    /* launch substeps eagerly instead of replaying a hipGraph */
 #define VFS_FLAG_STORE_ACCEL 0x4u /* keep the per-particle acceleration buffer up to date */
+/* After the substeps of each vfs_sim_step / vfs_sim_substeps call, check the
+ * state (vfs_sim_check_state with a margin of the box's largest extent) and
+ * return VFS_ERR_DIVERGED when a particle has a non-finite position or
+ * velocity or has left the domain.  The call then waits for its substeps.
+ * Single-scene simulations only. */
+#define VFS_FLAG_CHECK_STATE 0x8u
 
 /* Neighbour-list entries kept per particle and substep (a list longer than
  * this falls back to walking the cells, with identical results). */
@@ -197,6 +204,14 @@ int vfs_sim_profile_stages(vfs_sim *sim, uint32_t n, float *ms, uint32_t *launch
  * capacity of VFS_NEIGHBOUR_CAPACITY entries (their forces walk the cells).
  * No counterpart in the reference (it has no neighbour lists). */
 int vfs_sim_neighbour_stats(vfs_sim *sim, double *mean, uint32_t *max_count, uint32_t *overflowed);
+/* Counts, on the state after the work queued so far, the particles with a
+ * non-finite position or velocity component and the particles farther than
+ * `margin` (>= 0) outside the bounding box on any axis (the box walls are a
+ * penalty force, so a stable run keeps particles within a fraction of the
+ * smoothing radius of it).  Blocks until the count is done.  No counterpart in
+ * the reference (it has no state check; VK_CHECK aborts on API errors only,
+ * src/gfx/common.h:20-29). */
+int vfs_sim_check_state(vfs_sim *sim, float margin, uint32_t *non_finite, uint32_t *outside);
 /* Keys that changed bucket in the last substep: the keys its re-sort sorted
  * and merged (n when that substep sorted every key).  No counterpart in the
  * reference (it sorts every key every substep, sort.cpp:107-143). */

@@ -121,3 +121,14 @@ def test_render_buffer_argument_checks():
     rc, msg = _err(V.lib.vfs_render_buffers_export, C.byref(b), C.byref(V._lib.RenderIpc()))
     assert rc == -1 and "2 GiB" in msg
     assert V.lib.vfs_render_buffers_destroy(None) == 0
+
+
+def test_state_check_flag_is_for_single_scenes():
+    p = V.make_params(10, 0.04, dt=1e-4, flags=V.VFS_FLAG_CHECK_STATE)
+    arr = (V.Params * 2)(p, p)
+    counts = (C.c_uint32 * 2)(0, 0)
+    out = C.c_void_p()
+    rc, msg = _err(V.lib.vfs_batch_create, arr, 2, None, counts, C.byref(out))
+    assert rc == -1 and "VFS_FLAG_CHECK_STATE" in msg
+    rc, msg = _err(V.lib.vfs_sim_check_state, None, 0.0, None, None)
+    assert rc == -1 and "null" in msg

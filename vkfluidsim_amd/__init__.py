@@ -12,12 +12,14 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import (BoundaryObject, DeviceArray, LagueParams, Params, Tuning, VfsError, VFS_FLAG_NO_GRAPH, VFS_FLAG_STORE_ACCEL,
+from ._lib import (BoundaryObject, DeviceArray, LagueParams, Params, Tuning, VfsError, VFS_FLAG_CHECK_STATE,
+                   VFS_FLAG_NO_GRAPH, VFS_FLAG_STORE_ACCEL,
                    VFS_FLAG_TRACK_IDS, VFS_NEIGHBOUR_CAPACITY, VFS_SHAPE_BOX, VFS_SHAPE_BOX_WALLS, VFS_SHAPE_SPHERE, call, fptr, lib, uptr)
 
 __all__ = ["Sim", "Batch", "Adjoint", "Params", "Tuning", "BoundaryGrid", "SpatialHash", "VfsError", "make_params", "volume_map",
            "sort_pairs", "scan_exclusive", "spatial_offsets", "reorder_f32x3", "Scene", "lib",
-           "VFS_FLAG_TRACK_IDS", "VFS_FLAG_NO_GRAPH", "VFS_FLAG_STORE_ACCEL", "VFS_NEIGHBOUR_CAPACITY",
+           "VFS_FLAG_TRACK_IDS", "VFS_FLAG_NO_GRAPH", "VFS_FLAG_STORE_ACCEL", "VFS_FLAG_CHECK_STATE",
+           "VFS_NEIGHBOUR_CAPACITY",
            "VFS_SHAPE_BOX", "VFS_SHAPE_SPHERE", "VFS_SHAPE_BOX_WALLS", "device_count"]
 
 device_count = _lib.device_count
@@ -401,6 +403,12 @@ class Sim:
         mean, mx, over = C.c_double(0), C.c_uint32(0), C.c_uint32(0)
         call("vfs_sim_neighbour_stats", self.handle, C.byref(mean), C.byref(mx), C.byref(over))
         return mean.value, mx.value, over.value
+
+    def check_state(self, margin: float) -> tuple:
+        """(particles with a non-finite x or v, particles farther than margin outside the box)."""
+        bad, out = C.c_uint32(0), C.c_uint32(0)
+        call("vfs_sim_check_state", self.handle, float(margin), C.byref(bad), C.byref(out))
+        return int(bad.value), int(out.value)
 
     def changed_keys(self) -> int:
         """Keys that changed bucket in the last substep (n after a full sort)."""

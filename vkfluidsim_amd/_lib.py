@@ -17,6 +17,7 @@ LIB_PATH = PKG_DIR / "libvfs.so"
 VFS_FLAG_TRACK_IDS = 0x1
 VFS_FLAG_NO_GRAPH = 0x2
 VFS_FLAG_STORE_ACCEL = 0x4
+VFS_FLAG_CHECK_STATE = 0x8
 VFS_NEIGHBOUR_CAPACITY = 64
 
 VFS_SHAPE_BOX = 0
@@ -24,7 +25,7 @@ VFS_SHAPE_SPHERE = 1
 VFS_SHAPE_BOX_WALLS = 2
 
 STATUS = {0: "VFS_OK", -1: "VFS_ERR_INVALID", -2: "VFS_ERR_HIP", -3: "VFS_ERR_STATE",
-          -4: "VFS_ERR_IO", -5: "VFS_ERR_NOMEM"}
+          -4: "VFS_ERR_IO", -5: "VFS_ERR_NOMEM", -6: "VFS_ERR_DIVERGED"}
 
 
 class VfsError(RuntimeError):
@@ -164,6 +165,7 @@ SIGNATURES = {
     "vfs_sim_profile_stages": (_i32, [_vp, _u32, _fp, _up, C.POINTER(C.c_char_p), _u32, _up]),
     "vfs_sim_neighbour_stats": (_i32, [_vp, C.POINTER(C.c_double), _up, _up]),
     "vfs_sim_sort_stats": (_i32, [_vp, _up]),
+    "vfs_sim_check_state": (_i32, [_vp, C.c_float, _up, _up]),
     "vfs_batch_create": (_i32, [_vp, _u32, _vp, _up, C.POINTER(_vp)]),
     "vfs_batch_destroy": (_i32, [_vp]),
     "vfs_batch_info": (_i32, [_vp, _up, C.POINTER(C.c_uint64), _up]),

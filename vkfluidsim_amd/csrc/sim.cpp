@@ -19,6 +19,7 @@
 // its scene's constants from a table.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -134,7 +135,7 @@ void free_sim(vfs_sim* s) {
     void* ptrs[] = {s->x,           s->v,       s->xs,         s->vs,       s->bnd,      s->accel,
                     s->rho,         s->pid,     s->pids,       s->keys,     s->sorted_keys, s->perm,
                     s->offsets,     s->cell_start, s->sort_ws, s->objs_dev, s->nbr,      s->nbr_count,
-                    s->consts_dev,  s->wave_scene, s->pred, s->ps, s->near_rho};
+                    s->consts_dev,  s->wave_scene, s->pred, s->ps, s->near_rho, s->check_counters};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (float* g : s->grids)
@@ -218,6 +219,8 @@ int create_impl(const vfs_params* params, uint32_t n_scenes, const vfs_boundary_
     for (uint32_t k = 0; k < n_scenes; k++) {
         VFS_TRY(validate_params(&params[k]));
         VFS_CHECK(params[k].flags == params[0].flags, VFS_ERR_INVALID, "all scenes of a batch need the same flags");
+        VFS_CHECK(!batch || !(params[k].flags & VFS_FLAG_CHECK_STATE), VFS_ERR_INVALID,
+                  "VFS_FLAG_CHECK_STATE is for single-scene simulations");
         const uint64_t np = (uint64_t)params[k].n_particles;
         total += batch ? (np + 63u) / 64u * 64u : np;
         total_objects += objects_per_scene[k];
@@ -269,6 +272,7 @@ int create_impl(const vfs_params* params, uint32_t n_scenes, const vfs_boundary_
     r |= dev_alloc(&s->nbr, (size_t)n * kNeighbourWords);
     r |= dev_alloc(&s->nbr_count, n);
     r |= dev_alloc(&s->cell_start, (size_t)n + 1);
+    r |= dev_alloc(&s->check_counters, 2);
     if (s->flags & VFS_FLAG_TRACK_IDS) {
         r |= dev_alloc(&s->pid, n);
         r |= dev_alloc(&s->pids, n);
@@ -763,10 +767,40 @@ int vfs_sim_get_params(const vfs_sim* sim, vfs_params* params) {
     return 0;
 }
 
+// Counts of the state after the work queued on the simulation stream.
+int check_state_impl(vfs_sim* sim, float margin, uint32_t* non_finite, uint32_t* outside) {
+    VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
+    VFS_CHECK(!sim->batch, VFS_ERR_INVALID, "the state check runs on single-scene simulations");
+    VFS_CHECK(margin >= 0.0f, VFS_ERR_INVALID, "margin must be >= 0");
+    const SimConsts& c = sim->scenes[0].consts;
+    VFS_TRY(launch_check_state(sim->x, sim->v, sim->n, c.bb_lo, c.bb_hi, margin, sim->check_counters, sim->stream));
+    uint32_t counts[2] = {0, 0};
+    VFS_HIP(hipMemcpyAsync(counts, sim->check_counters, sizeof(counts), hipMemcpyDeviceToHost, sim->stream));
+    VFS_HIP(hipStreamSynchronize(sim->stream));
+    if (non_finite) *non_finite = counts[0];
+    if (outside) *outside = counts[1];
+    return 0;
+}
+
 int vfs_sim_substeps(vfs_sim* sim, uint32_t n, void* stream) {
     VFS_CHECK(sim, VFS_ERR_INVALID, "sim is null");
     hipStream_t s = stream ? as_stream(stream) : sim->stream;
-    return sim->substeps(n, s);
+    VFS_TRY(sim->substeps(n, s));
+    if (!(sim->flags & VFS_FLAG_CHECK_STATE) || n == 0) return 0;
+    // a particle farther than the box's largest extent outside it has left the domain
+    const float* size = sim->scenes[0].params.bbox_size;
+    const float margin = std::max(size[0], std::max(size[1], size[2]));
+    uint32_t bad = 0, out = 0;
+    VFS_TRY(check_state_impl(sim, margin, &bad, &out));
+    if (bad || out)
+        return fail(VFS_ERR_DIVERGED, "state check after " + std::to_string(n) + " substeps: " +
+                                          std::to_string(bad) + " particles with a non-finite position or velocity, " +
+                                          std::to_string(out) + " outside the domain");
+    return 0;
+}
+
+int vfs_sim_check_state(vfs_sim* sim, float margin, uint32_t* non_finite, uint32_t* outside) {
+    return check_state_impl(sim, margin, non_finite, outside);
 }
 
 int vfs_sim_synchronize(vfs_sim* sim) {

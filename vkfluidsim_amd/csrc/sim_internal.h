@@ -36,6 +36,7 @@ struct vfs_sim {
     float4 *x = nullptr, *v = nullptr, *xs = nullptr, *vs = nullptr, *bnd = nullptr, *accel = nullptr;
     float* rho = nullptr;
     uint32_t *nbr = nullptr, *nbr_count = nullptr;
+    uint32_t* check_counters = nullptr;  // state check: non-finite, escaped particles
     uint32_t *pid = nullptr, *pids = nullptr, *keys = nullptr, *sorted_keys = nullptr, *perm = nullptr,
              *offsets = nullptr, *cell_start = nullptr, *sort_ws = nullptr;
     SimConsts* consts_dev = nullptr;  // batch: scene table (+ one empty scene)

@@ -118,6 +118,10 @@ struct StepBuffers {
 constexpr uint32_t kCandidateGroup = 4;
 
 int launch_reorder_state(const StepBuffers& b, hipStream_t s);
+// counters[0] = particles with a non-finite x or v component, counters[1] =
+// particles farther than margin outside [lo, hi] (counters cleared first).
+int launch_check_state(const float4* x, const float4* v, uint32_t n, const float* lo, const float* hi, float margin,
+                       uint32_t* counters, hipStream_t s);
 // Lague model stages (one scene): predictions and their hash keys, density
 // and near density, pressure + viscosity + integration + collisions.
 int launch_lague_predict(const StepBuffers& b, const SimConsts& c, const LagueConsts& lc, hipStream_t s);

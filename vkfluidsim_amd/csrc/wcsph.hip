@@ -722,6 +722,42 @@ inline uint32_t particle_blocks(uint32_t n, uint32_t block) { return (grid_for(n
 
 }  // namespace
 
+// State check: particles with a non-finite position or velocity component
+// (counters[0]) and particles farther than `margin` outside [lo, hi] on any
+// axis (counters[1]); one atomic per wave.  The first kernel clears the
+// counters (a kernel, not a memset, so the pair can sit in a captured graph).
+__global__ void k_check_clear(uint32_t* counters) {
+    if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_check_state(const float4* __restrict__ x, const float4* __restrict__ v,
+                                                     uint32_t n, float3 lo, float3 hi, float margin,
+                                                     uint32_t* __restrict__ counters) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    bool bad = false, out = false;
+    if (i < n) {
+        const float4 p = x[i], q = v[i];
+        bad = !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(q.x) && isfinite(q.y) && isfinite(q.z));
+        out = !bad && (p.x < lo.x - margin || p.y < lo.y - margin || p.z < lo.z - margin || p.x > hi.x + margin ||
+                       p.y > hi.y + margin || p.z > hi.z + margin);
+    }
+    const uint32_t nb = (uint32_t)__popcll(__ballot(bad)), no = (uint32_t)__popcll(__ballot(out));
+    if ((threadIdx.x & 63u) == 0u) {
+        if (nb) atomicAdd(&counters[0], nb);
+        if (no) atomicAdd(&counters[1], no);
+    }
+}
+
+int launch_check_state(const float4* x, const float4* v, uint32_t n, const float* lo, const float* hi, float margin,
+                       uint32_t* counters, hipStream_t s) {
+    hipLaunchKernelGGL(k_check_clear, dim3(1), dim3(64), 0, s, counters);
+    VFS_TRY(check_launch("check_clear"));
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_check_state, dim3((n + 255u) / 256u), dim3(256), 0, s, x, v, n,
+                       make_float3(lo[0], lo[1], lo[2]), make_float3(hi[0], hi[1], hi[2]), margin, counters);
+    return check_launch("check_state");
+}
+
 int launch_reorder_state(const StepBuffers& b, hipStream_t s) {
     if (b.n == 0) return 0;
     const uint32_t nb = particle_blocks(b.n, 256);
