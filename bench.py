@@ -503,6 +503,45 @@ def summarize_stats(db: str) -> str:
     return "\n".join(lines)
 
 
+SQ_COUNTER_PASSES = (
+    "SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_ANY SQ_ACTIVE_INST_VALU "
+    "SQ_ACTIVE_INST_LDS",
+    "SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_LDS_BANK_CONFLICT "
+    "SQ_WAIT_INST_LDS SQ_INSTS_BRANCH",
+    "SQ_ACTIVE_INST_SCA SQ_ACTIVE_INST_VMEM SQ_ACTIVE_INST_MISC SQ_INST_CYCLES_VMEM_RD SQ_LDS_IDX_ACTIVE "
+    "SQ_LDS_ADDR_CONFLICT SQ_IFETCH SQ_THREAD_CYCLES_VALU",
+)
+
+
+def summarize_counters(dbs: list) -> str:
+    """Per-wave SQ counters of the density and force kernels from rocprofv3 --pmc
+    databases of --traffic-probe runs (one database per pass of
+    SQ_COUNTER_PASSES), with the VALU issue fraction against the kernel's
+    duration from the same runs."""
+    import sqlite3
+
+    per = {}
+    for db in dbs:
+        con = sqlite3.connect(db)
+        for name, counter, value in con.execute("select kernel_name, counter_name, value from counters_collection"):
+            per.setdefault(kernel_base(name), {}).setdefault(counter, []).append(value)
+    lines = []
+    for k in ("k_density", "k_forces_integrate"):
+        if k not in per or "SQ_WAVES" not in per[k]:
+            continue
+        mean = {c: sum(v) / len(v) for c, v in per[k].items()}
+        waves = mean["SQ_WAVES"]
+        lines.append(f"{k}: {waves:.0f} waves per launch (mean of {len(per[k]['SQ_WAVES'])} launches)")
+        for c in sorted(mean):
+            if c == "SQ_WAVES":
+                continue
+            lines.append(f"  {c:26s} {mean[c]:14.4g}   per wave {mean[c] / waves:10.1f}")
+        if "SQ_THREAD_CYCLES_VALU" in mean and "SQ_ACTIVE_INST_VALU" in mean:
+            lines.append(f"  lanes active per VALU instruction: "
+                         f"{mean['SQ_THREAD_CYCLES_VALU'] / (64.0 * mean['SQ_ACTIVE_INST_VALU']):.3f}")
+    return "\n".join(lines)
+
+
 def summarize_pmc(fetch_db: str, write_db: str) -> str:
     """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of two rocprofv3
     --pmc databases of the traffic probe, and the HBM bytes per launch after
@@ -662,12 +701,17 @@ def main() -> int:
     ap.add_argument("--summarize-stats", metavar="DB", help="print a rocprofv3 --stats database as text")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="print the HBM traffic of two --pmc databases of --traffic-probe runs")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="DB",
+                    help="print the per-wave SQ counters of --pmc databases of --traffic-probe runs")
     args = ap.parse_args()
     if args.summarize_stats:
         print(summarize_stats(args.summarize_stats))
         return 0
     if args.summarize_pmc:
         print(summarize_pmc(*args.summarize_pmc))
+        return 0
+    if args.summarize_counters:
+        print(summarize_counters(args.summarize_counters))
         return 0
 
     if args.gpus != 1:
