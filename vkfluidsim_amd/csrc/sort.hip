@@ -679,6 +679,39 @@ __global__ __launch_bounds__(256) void k_resort_compact(const uint32_t* __restri
     }
 }
 
+// First index in [lo, hi) of the sorted changed keys with a key >= k (hi if none).
+__device__ __forceinline__ uint32_t changed_lower_bound(const uint32_t* __restrict__ bks, uint32_t k, uint32_t lo,
+                                                        uint32_t hi) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (bks[mid] < k)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// First index in [lo, hi) where pred is false, pred being true on a prefix of
+// the range; the whole wave calls it with the same arguments.  Each round the
+// 64 lanes test the last index of 64 equal chunks, and the count of true tests
+// picks the chunk holding the answer: a range of c entries takes about
+// log64(c) + 1 dependent loads instead of log2(c).
+template <class Pred>
+__device__ __forceinline__ uint32_t wave_partition_point(uint32_t lo, uint32_t hi, Pred&& pred) {
+    const uint32_t lane = threadIdx.x & 63u;
+    while (hi - lo > 64u) {
+        const uint32_t step = (hi - lo + 63u) >> 6;
+        const uint32_t probe = lo + (lane + 1u) * step - 1u;
+        const bool t = probe < hi && pred(probe);
+        lo += (uint32_t)__popcll(__ballot(t)) * step;  // a true test is below hi, so lo <= hi
+        hi = min(hi, lo + step);
+    }
+    const uint32_t probe = lo + lane;
+    const bool t = probe < hi && pred(probe);
+    return lo + (uint32_t)__popcll(__ballot(t));
+}
+
 // Output positions of both runs: threads below the changed count place the
 // sorted changed entry of their index, every unchanged index finds how many
 // changed entries precede it by binary search.  MOVE: each placed entry also
@@ -750,23 +783,32 @@ __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict
 // Bucket starts of the re-sorted keys, in place over the previous ones: the
 // entries with a key below k are the unchanged entries at previous indices
 // below cs[k] (cs[k] - nb[cs[k]] of them) and the changed keys below k (a
-// lower bound in the sorted changed keys).  m = table size + 1 entries.
+// lower bound in the sorted changed keys).  The lower bounds of the block's
+// keys lie between those of its first key and of the key after its last, so
+// waves 0 and 1 find those two among all changed keys (a wave-wide search)
+// and every thread searches only that window.
+// m = table size + 1 entries.
 __global__ __launch_bounds__(256) void k_resort_cell_starts(uint32_t* __restrict__ cs, const uint32_t* __restrict__ nb,
                                                             const uint32_t* __restrict__ bks,
                                                             const uint32_t* __restrict__ state, uint32_t m) {
-    const uint32_t k = blockIdx.x * (uint32_t)kThreads + threadIdx.x;
-    if (k >= m) return;
+    __shared__ uint32_t s_window[2];
+    const uint32_t k0 = blockIdx.x * (uint32_t)kThreads;
+    const uint32_t k = k0 + threadIdx.x;
     const uint32_t count = state[0];
-    const uint32_t c = cs[k];
-    uint32_t lo = 0, hi = count;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (bks[mid] < k)
-            lo = mid + 1u;
-        else
-            hi = mid;
+    uint32_t below = 0u;
+    if (k < m) {
+        const uint32_t c = cs[k];
+        below = c - nb[c];
     }
-    cs[k] = (c - nb[c]) + lo;
+    const uint32_t wave = threadIdx.x >> 6;
+    if (wave < 2u) {
+        const uint32_t kp = wave == 0u ? k0 : k0 + (uint32_t)kThreads;
+        const uint32_t r = wave_partition_point(0u, count, [&](uint32_t i) { return bks[i] < kp; });
+        if ((threadIdx.x & 63u) == 0u) s_window[wave] = r;
+    }
+    __syncthreads();
+    if (k >= m) return;
+    cs[k] = below + changed_lower_bound(bks, k, s_window[0], s_window[1]);
 }
 
 inline uint32_t grid_for(uint64_t n, uint32_t block = 256) {
