@@ -432,6 +432,44 @@ def test_resort_after_state_in_any_order():
     _assert_same_step(sim2, orc2)
 
 
+@pytest.mark.parametrize("swaps", [500, 2000, 2100])
+def test_resort_sorts_changed_keys_in_one_block_up_to_4096(swaps):
+    """The keys that changed bucket are sorted by one block in LDS when there
+    are at most 4,096 of them and by the multi-tile radix passes above that.
+    Resting particles (no pressure, gravity or viscosity, so nothing moves)
+    are swapped pairwise after their first substep: exactly two keys per swap
+    of particles in different buckets differ from the previous sorted keys at
+    their index (about 1,000, 4,000 and 4,200 here), and the next substeps
+    match the oracle bit for bit on both sides of the capacity."""
+    rng = np.random.default_rng(swaps)
+    n = 60_000
+    xs = rng.uniform(0.05, 0.95, size=(n, 3)).astype(np.float32)
+    vs = np.zeros((n, 3), np.float32)
+    p = V.make_params(n, 0.03, dt=1e-4, box_size=(1.0, 1.0, 1.0), gravity=(0.0, 0.0, 0.0), viscosity=0.0,
+                      stiffness=0.0, flags=V.VFS_FLAG_TRACK_IDS)
+    sim = V.Sim(p, [])
+    orc = H.OracleSim(p, [])
+    for s in (sim, orc):
+        s.set_state(xs, vs)
+        s.substeps(1)
+    keys, _, _ = sim.get_hash()
+    x, v, _, _ = sim.get_state()
+    pick = rng.permutation(n)[: 2 * swaps].reshape(-1, 2)
+    pick = pick[keys[pick[:, 0]] != keys[pick[:, 1]]]
+    perm = np.arange(n)
+    perm[pick[:, 0]], perm[pick[:, 1]] = pick[:, 1], pick[:, 0]
+    for s in (sim, orc):
+        s.set_state(x[perm], v[perm])
+        s.substeps(1)
+    print(f"case {swaps} swaps: {sim.changed_keys()} changed keys", flush=True)
+    assert sim.changed_keys() == 2 * len(pick)
+    _assert_same_step(sim, orc)
+    sim.substeps(2)
+    orc.substeps(2)
+    _assert_same_step(sim, orc)
+    sim.close()
+
+
 def test_resort_every_key_changed_at_300k():
     """The multi-tile re-sort with nearly every key changed: 300,001 particles
     (a ragged count) shuffled after their first substeps, so that almost every
