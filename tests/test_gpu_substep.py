@@ -432,15 +432,15 @@ def test_resort_after_state_in_any_order():
     _assert_same_step(sim2, orc2)
 
 
-@pytest.mark.parametrize("swaps", [500, 2000, 2100])
+@pytest.mark.parametrize("swaps", [500, 2000, 2100, 8300])
 def test_resort_sorts_changed_keys_in_one_block_up_to_4096(swaps):
     """The keys that changed bucket are sorted by one block in LDS when there
     are at most 4,096 of them and by the multi-tile radix passes above that.
     Resting particles (no pressure, gravity or viscosity, so nothing moves)
     are swapped pairwise after their first substep: exactly two keys per swap
     of particles in different buckets differ from the previous sorted keys at
-    their index (about 1,000, 4,000 and 4,200 here), and the next substeps
-    match the oracle bit for bit on both sides of the capacity."""
+    their index (about 1,000, 4,000, 4,200 and 16,600 here), and the next
+    substeps match the oracle bit for bit on both sides of the capacity."""
     rng = np.random.default_rng(swaps)
     n = 60_000
     xs = rng.uniform(0.05, 0.95, size=(n, 3)).astype(np.float32)
