@@ -1,6 +1,8 @@
-"""The reference's own obstacle scene (assets/scenes/obstacle.json with
-models/box.obj and models/suzanne.obj, committed as data under
-tests/golden/reference_scene/) loaded through the C ABI: the JSON transforms
+"""The reference's own scenes, committed as data under tests/golden/reference_scene/:
+the dam break (assets/scenes/dam_break.json: parameters, the reference's lattice
+seeding, frames against the oracle bit for bit) and the obstacle scene
+(assets/scenes/obstacle.json with models/box.obj and models/suzanne.obj) loaded
+through the C ABI: the JSON transforms
 against hand-computed known answers and an independent T R S composition
 (scene_loader.cpp:85-105, transform.cpp:34-46, generic_scene.cpp:134-136),
 the OBJ triangles against a plain parse of the file, the SDF and volume-map
@@ -16,6 +18,7 @@ import pytest
 
 import harness as H
 import vkfluidsim_amd as V
+from vkfluidsim_amd import configs as CF
 
 pytestmark = pytest.mark.gpu
 
@@ -158,3 +161,30 @@ class _SimView:
     def __init__(self, scene):
         self.handle = scene.sim_handle
         self.n = scene.n
+
+
+def test_reference_dam_break_scene_loads_seeds_and_steps_like_the_oracle():
+    cfg = json.loads((ROOT / "dam_break.json").read_text())
+    sc = V.Scene(str(ROOT / "dam_break.json"), str(ROOT))
+    p = sc.params
+    assert sc.n == 30 * 50 * 50
+    assert p.smooth_radius == pytest.approx(0.2) and p.iterations == 3 and p.fixed_dt == pytest.approx(0.008333)
+    assert p.stiffness == 1000.0 and p.expoent == 7.0 and p.viscosity_strenght == pytest.approx(0.01)
+    assert list(p.bbox_size) == [10.0, 10.0, 8.0] and list(p.gravity) == pytest.approx([0.0, -9.81, 0.0])
+    assert sc.boundary_objects() == [] and cfg["boundaryObjects"] == []
+    x0, v0, _, _ = sc.get_state()
+    block = cfg["fluidBlocks"][0]
+    np.testing.assert_array_equal(x0, CF.compact_block(tuple(block["size"]), 1000.0, pos=tuple(block["pos"])))
+    assert not v0.any()
+    orc = H.OracleSim(sc.params, [])
+    orc.set_state(x0, v0)
+    for frame in range(5):
+        sc.step()  # one frame: 3 substeps
+        orc.substeps(3)
+        print(f"frame {frame}", flush=True)
+        x, v, rho, _ = sc.get_state()
+        rx, rv, rrho, _, _ = orc.get_state()
+        np.testing.assert_array_equal(x.astype(np.float64), rx)
+        np.testing.assert_array_equal(v.astype(np.float64), rv)
+        np.testing.assert_array_equal(rho.astype(np.float64), rrho)
+    sc.close()

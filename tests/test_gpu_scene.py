@@ -40,11 +40,10 @@ def test_compact_lattice_seeding_like_the_reference():
     sc = V.Scene(str(SCENES / "compact_block.json"))
     assert sc.n == 10 * 12 * 14
     x0 = sc.get_state()[0]
-    d = np.cbrt(np.float32(1.0) / np.float32(1000.0)).astype(np.float32)
-    i, j, k = np.meshgrid(np.arange(10), np.arange(12), np.arange(14), indexing="ij")
-    idx = np.stack([i.reshape(-1), j.reshape(-1), k.reshape(-1)], 1).astype(np.float32)
-    want = d * idx + np.array([0.0, 0.0, 1.5], np.float32)
-    np.testing.assert_allclose(x0, want, rtol=0, atol=1e-6)
+    # the lattice as GenericScene::Reset sizes it (generic_scene.cpp:42-57): float
+    # spacing from the C library's cbrtf, one multiply and one add per coordinate
+    want = CF.compact_block((10, 12, 14), 1000.0, pos=(0.0, 0.0, 1.5))
+    np.testing.assert_array_equal(x0, want)
     sc.step()
     x1 = sc.get_state()[0]
     assert np.isfinite(x1).all() and not np.array_equal(x0, x1)
