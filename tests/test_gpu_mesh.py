@@ -80,7 +80,8 @@ def test_box_mesh_grids_match_analytic_box_on_gpu():
 
 def test_scene_with_obj_boundary_objects(tmp_path):
     """A scene in the layout of the reference's obstacle scene: a rotated and
-    scaled box OBJ (quads) and an icosphere OBJ, with resource paths."""
+    scaled box OBJ (quads) and an icosphere OBJ, with resource paths; 20
+    frames bit-exact against the oracle driven by the scene's own objects."""
     models = tmp_path / "models"
     models.mkdir()
     bv, _ = CF.box_mesh((0.0, 0.0, 0.0), (0.5, 0.5, 0.5))
@@ -107,8 +108,20 @@ def test_scene_with_obj_boundary_objects(tmp_path):
     path.write_text(json.dumps(scene))
     s = V.Scene(str(path), str(tmp_path))
     assert s.n == 20 * 10 * 16
-    for _ in range(20):
+    # the loaded objects (grids built on the GPU, transforms from the JSON) drive
+    # the oracle, which must then follow the scene bit for bit
+    x0, v0, _, _ = s.get_state()
+    orc = H.OracleSim(s.params, s.boundary_objects())
+    orc.set_state(x0, v0)
+    for frame in range(20):
         s.step()
+        orc.substeps(int(s.params.iterations))
+        if frame % 5 == 4:
+            x, v, rho, _ = s.get_state()
+            rx, rv, rrho, _, _ = orc.get_state()
+            np.testing.assert_array_equal(x.astype(np.float64), rx)
+            np.testing.assert_array_equal(v.astype(np.float64), rv)
+            np.testing.assert_array_equal(rho.astype(np.float64), rrho)
     x, v, rho, _ = s.get_state()
     assert np.isfinite(x).all() and np.isfinite(v).all() and np.isfinite(rho).all()
     s.close()
