@@ -71,3 +71,36 @@ def test_cpp_facade_matches_the_c_abi(tmp_path):
     np.testing.assert_array_equal(v, ev)
     np.testing.assert_array_equal(rho, er)
     np.testing.assert_array_equal(ids, eid)
+
+
+def test_two_fluid_blocks_are_seeded_one_after_the_other(tmp_path):
+    """GenericScene::Reset (generic_scene.cpp:42-57): each block's compact lattice
+    takes the next compMul(size) particles, in the order of the JSON."""
+    import json
+
+    blocks = [{"size": [6, 5, 4], "pos": [0.5, 0.0, 0.25]}, {"size": [3, 7, 2], "pos": [4.0, 1.0, 2.0]}]
+    scene = {
+        "name": "Two blocks", "fluidModel": "wcsph",
+        "wcsphParameters": {"stiffness": 1000.0, "expoent": 7.0, "viscosityStrenght": 0.01,
+                            "outerWallStiffness": 10000.0},
+        "simulationParameters": {"gravity": [0.0, -9.81, 0.0], "smoothRadius": 0.2, "timeScale": 1.0,
+                                 "iterations": 2, "dt": 0.004, "targetDensity": 800.0,
+                                 "boundingBox": {"pos": [0, 0, 0], "size": [6, 4, 4]}},
+        "fluidBlocks": blocks,
+        "boundaryObjects": [],
+    }
+    path = tmp_path / "two_blocks.json"
+    path.write_text(json.dumps(scene))
+    sc = V.Scene(str(path))
+    assert sc.n == 6 * 5 * 4 + 3 * 7 * 2
+    want = np.concatenate([CF.compact_block(tuple(b["size"]), 800.0, pos=tuple(b["pos"])) for b in blocks])
+    x0, v0, _, _ = sc.get_state()
+    np.testing.assert_array_equal(x0, want)
+    orc = H.OracleSim(sc.params, [])
+    orc.set_state(x0, v0)
+    sc.step()
+    orc.substeps(2)
+    np.testing.assert_array_equal(sc.get_state()[0].astype(np.float64), orc.get_state()[0])
+    sc.reset()
+    np.testing.assert_array_equal(sc.get_state()[0], want)
+    sc.close()
