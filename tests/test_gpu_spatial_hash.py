@@ -130,3 +130,17 @@ def test_spatial_hash_uniform_8m():
     assert np.array_equal(keys, rk)
     assert np.array_equal(perm, rp)
     assert np.array_equal(offs, H.oracle_offsets(rk))
+
+
+def test_sort_pairs_beyond_2_to_the_28_keys():
+    """The largest sizes: 2^28 + 12,345 keys (tile counts, indices and byte
+    offsets past the particle limit of a simulation) with 16-bit keys, against
+    numpy's stable sort of the same keys."""
+    n = (1 << 28) + 12345
+    print(f"case sort: {n} keys", flush=True)
+    rng = np.random.default_rng(28)
+    keys16 = rng.integers(0, 1 << 16, n, dtype=np.uint16)
+    sk, perm = V.sort_pairs(keys16.astype(np.uint32), max_value=(1 << 16) - 1)
+    ref_perm = np.argsort(keys16, kind="stable")
+    assert np.array_equal(perm, ref_perm)
+    assert np.array_equal(sk, keys16[ref_perm])
