@@ -511,6 +511,13 @@ SQ_COUNTER_PASSES = (
     "SQ_ACTIVE_INST_SCA SQ_ACTIVE_INST_VMEM SQ_ACTIVE_INST_MISC SQ_INST_CYCLES_VMEM_RD SQ_LDS_IDX_ACTIVE "
     "SQ_LDS_ADDR_CONFLICT SQ_IFETCH SQ_THREAD_CYCLES_VALU",
 )
+# The vector memory path of the same kernels: texture address (TA) and data
+# (TD) units and the L1 cache (TCP) of every CU, summed over the 256 CUs, with
+# the kernel's cycles (GRBM_GUI_ACTIVE counts each of the 8 XCDs).
+MEMORY_COUNTER_PASS = ("GRBM_GUI_ACTIVE TA_TA_BUSY_sum TA_BUFFER_READ_WAVEFRONTS_sum TD_TD_BUSY_sum TD_TC_STALL_sum "
+                       "TCP_TOTAL_CACHE_ACCESSES_sum TCP_TCC_READ_REQ_sum TCP_PENDING_STALL_CYCLES_sum "
+                       "TCP_TCR_TCP_STALL_CYCLES_sum")
+CUS, XCDS = 256, 8
 
 
 def summarize_counters(dbs: list) -> str:
@@ -539,6 +546,21 @@ def summarize_counters(dbs: list) -> str:
         if "SQ_THREAD_CYCLES_VALU" in mean and "SQ_ACTIVE_INST_VALU" in mean:
             lines.append(f"  lanes active per VALU instruction: "
                          f"{mean['SQ_THREAD_CYCLES_VALU'] / (64.0 * mean['SQ_ACTIVE_INST_VALU']):.3f}")
+        if "GRBM_GUI_ACTIVE" in mean:
+            cyc = mean["GRBM_GUI_ACTIVE"] / XCDS
+            lines.append(f"  kernel cycles (GRBM_GUI_ACTIVE / {XCDS} XCDs): {cyc:.0f}")
+            if "SQ_ACTIVE_INST_VALU" in mean:
+                lines.append(f"  VALU busy (4 x SQ_ACTIVE_INST_VALU / {SIMDS} SIMDs / kernel cycles): "
+                             f"{4.0 * mean['SQ_ACTIVE_INST_VALU'] / SIMDS / cyc:.3f}")
+            for c in ("TA_TA_BUSY_sum", "TD_TD_BUSY_sum", "TD_TC_STALL_sum", "TCP_PENDING_STALL_CYCLES_sum"):
+                if c in mean:
+                    lines.append(f"  {c} per CU / kernel cycles: {mean[c] / CUS / cyc:.3f}")
+            if "SQ_INSTS_VMEM_RD" in mean:
+                lines.append(f"  TD data cycles (16 per wave64 16-byte load, SQ_INSTS_VMEM_RD) per CU / kernel cycles: "
+                             f"{16.0 * mean['SQ_INSTS_VMEM_RD'] / CUS / cyc:.3f}")
+            if "TCP_TOTAL_CACHE_ACCESSES_sum" in mean and "TCP_TCC_READ_REQ_sum" in mean:
+                lines.append(f"  L1 hit rate (1 - TCP_TCC_READ_REQ / TCP_TOTAL_CACHE_ACCESSES): "
+                             f"{1.0 - mean['TCP_TCC_READ_REQ_sum'] / mean['TCP_TOTAL_CACHE_ACCESSES_sum']:.3f}")
     return "\n".join(lines)
 
 

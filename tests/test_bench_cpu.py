@@ -78,3 +78,22 @@ def test_traffic_summary_scales_by_the_calibration_copy(tmp_path):
     assert row[1] == "2" and float(row[2]) == 2000.0 and float(row[3]) == 500.0
     # (2000 * 2 + 500 * 1) KiB per launch = 4.608 MB
     assert float(row[4]) == pytest.approx((2000.0 * 2.0 + 500.0) * 1024.0 / 1e6, abs=0.05)
+
+
+def test_counter_summary_memory_path_fractions(tmp_path):
+    # one launch: kernel cycles from GRBM_GUI_ACTIVE over 8 XCDs, per-CU busy fractions of
+    # the 256-CU sums, VALU busy from 4-cycle units over 1024 SIMDs, the L1 hit rate
+    name = "void vfs::(anonymous namespace)::k_forces_integrate<128, false>(vfs::StepBuffers, ...)"
+    cyc = 1000.0
+    rows = [(name, "SQ_WAVES", 64.0, 0), (name, "GRBM_GUI_ACTIVE", 8 * cyc, 0),
+            (name, "SQ_ACTIVE_INST_VALU", 0.5 * cyc * 1024 / 4, 0),
+            (name, "TD_TD_BUSY_sum", 0.8 * cyc * 256, 0), (name, "TD_TC_STALL_sum", 0.4 * cyc * 256, 0),
+            (name, "SQ_INSTS_VMEM_RD", 0.25 * cyc * 256 / 16, 0),
+            (name, "TCP_TOTAL_CACHE_ACCESSES_sum", 1000.0, 0), (name, "TCP_TCC_READ_REQ_sum", 200.0, 0)]
+    text = bench.summarize_counters([_counter_db(tmp_path / "m.db", rows)])
+    assert "kernel cycles (GRBM_GUI_ACTIVE / 8 XCDs): 1000" in text
+    assert "VALU busy (4 x SQ_ACTIVE_INST_VALU / 1024 SIMDs / kernel cycles): 0.500" in text
+    assert "TD_TD_BUSY_sum per CU / kernel cycles: 0.800" in text
+    assert "TD_TC_STALL_sum per CU / kernel cycles: 0.400" in text
+    assert "per CU / kernel cycles: 0.250" in text  # 16 TD cycles per VMEM read instruction
+    assert "L1 hit rate (1 - TCP_TCC_READ_REQ / TCP_TOTAL_CACHE_ACCESSES): 0.800" in text
