@@ -432,16 +432,17 @@ def test_resort_after_state_in_any_order():
     _assert_same_step(sim2, orc2)
 
 
-@pytest.mark.parametrize("swaps", [500, 2000, 2100, 8300])
-def test_resort_sorts_changed_keys_in_one_block_up_to_4096(swaps):
-    """The keys that changed bucket are sorted by one block in LDS when there
-    are at most 4,096 of them and by the multi-tile radix passes above that.
+@pytest.mark.parametrize("pairs", [500, 2048, 2049, 8192, 8193])
+def test_resort_sorts_changed_keys_by_rank_up_to_16384(pairs):
+    """The keys that changed bucket are ranked by counting when there are at
+    most 16,384 of them and sorted by the multi-tile radix passes above that.
     Resting particles (no pressure, gravity or viscosity, so nothing moves)
-    are swapped pairwise after their first substep: exactly two keys per swap
-    of particles in different buckets differ from the previous sorted keys at
-    their index (about 1,000, 4,000, 4,200 and 16,600 here), and the next
-    substeps match the oracle bit for bit on both sides of the capacity."""
-    rng = np.random.default_rng(swaps)
+    are swapped pairwise after their first substep, each pair from two
+    different buckets: exactly two keys per pair differ from the previous
+    sorted keys at their index (1,000, 4,096, 4,098, 16,384 and 16,386 here),
+    and the next substeps match the oracle bit for bit on both sides of the
+    capacity."""
+    rng = np.random.default_rng(pairs)
     n = 60_000
     xs = rng.uniform(0.05, 0.95, size=(n, 3)).astype(np.float32)
     vs = np.zeros((n, 3), np.float32)
@@ -454,15 +455,16 @@ def test_resort_sorts_changed_keys_in_one_block_up_to_4096(swaps):
         s.substeps(1)
     keys, _, _ = sim.get_hash()
     x, v, _, _ = sim.get_state()
-    pick = rng.permutation(n)[: 2 * swaps].reshape(-1, 2)
-    pick = pick[keys[pick[:, 0]] != keys[pick[:, 1]]]
+    pick = rng.permutation(n)[: 2 * (pairs + 2000)].reshape(-1, 2)
+    pick = pick[keys[pick[:, 0]] != keys[pick[:, 1]]][:pairs]
+    assert len(pick) == pairs
     perm = np.arange(n)
     perm[pick[:, 0]], perm[pick[:, 1]] = pick[:, 1], pick[:, 0]
     for s in (sim, orc):
         s.set_state(x[perm], v[perm])
         s.substeps(1)
-    print(f"case {swaps} swaps: {sim.changed_keys()} changed keys", flush=True)
-    assert sim.changed_keys() == 2 * len(pick)
+    print(f"case {pairs} pairs: {sim.changed_keys()} changed keys", flush=True)
+    assert sim.changed_keys() == 2 * pairs
     _assert_same_step(sim, orc)
     sim.substeps(2)
     orc.substeps(2)

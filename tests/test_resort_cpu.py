@@ -75,3 +75,34 @@ def test_a_wrong_clamp_is_caught():
     c = np.clip(bis, cs_new[bks], cs_new[bks + 1])
     wrong = (c - nb[c]) + np.arange(len(bis))
     assert not np.array_equal(wrong, out[bis])
+
+
+def _rank_by_counting(bkeys, waves=16):
+    """k_resort_rank's formula: the entry at compacted position p goes to
+    #{q : (key_q, q) < (key_p, p)}, the comparisons split over `waves` equal
+    shares of q whose partial counts are summed."""
+    c = len(bkeys)
+    comp = bkeys.astype(np.int64) * (1 << 32) + np.arange(c)
+    chunk = (c + waves - 1) // waves
+    rank = np.zeros(c, np.int64)
+    for w in range(waves):
+        q0, q1 = min(c, w * chunk), min(c, w * chunk + chunk)
+        rank += (comp[None, q0:q1] < comp[:, None]).sum(axis=1)
+    return rank
+
+
+@pytest.mark.parametrize("c,table", [(1, 5), (16, 3), (100, 1000), (1000, 40), (4097, 300), (16384, 1 << 20)])
+def test_rank_by_counting_is_the_stable_order(c, table):
+    """Compacted changed keys (index order) placed by their counting rank form
+    the (key, index) order the merge expects: np.lexsort's, ties by position."""
+    rng = np.random.default_rng(c)
+    bkeys = rng.integers(0, table, c).astype(np.uint32)
+    rank = _rank_by_counting(bkeys)
+    order = np.empty(c, np.int64)
+    order[rank] = np.arange(c)
+    np.testing.assert_array_equal(order, np.lexsort((np.arange(c), bkeys)))
+    # ties broken the other way (later position first) would not be stable
+    if len(np.unique(bkeys)) < c:
+        comp = bkeys.astype(np.int64) * (1 << 32) + (c - 1 - np.arange(c))
+        wrong = (comp[None, :] < comp[:, None]).sum(axis=1)
+        assert not np.array_equal(wrong, rank)

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
     __shared__ uint32_t s_vals[TILE];
     __shared__ uint32_t s_tmp[kWaves];
 
-    // a device-side count of at most small_cap keys was sorted by k_resort_small
+    // a device-side count of at most small_cap keys was sorted by k_resort_rank
     if (n_dev && *n_dev <= small_cap) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
@@ -247,111 +247,62 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
     }
 }
 
-// The whole stable LSD radix sort of at most kSmallSort (key, value) pairs
-// (count read on the device) in one block's LDS: each pass ranks the keys as
-// k_radix_pass ranks a tile (wave ballots; wave-major, item-major, lane order =
-// input order) and scatters them within LDS.  A larger count leaves at once
-// (the multi-tile passes sort it).  THREADS = 1024: sixteen waves of four
-// items each keep the CU's four SIMDs busy on the serial per-item ranking
-// (96 KB of LDS for 10-bit digits).
-constexpr uint32_t kSmallSort = 4096;
-constexpr int kSmallThreads = 1024;
-template <int BITS, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_resort_small(const uint32_t* __restrict__ keys_in,
-                                                          const uint32_t* __restrict__ vals_in,
-                                                          uint32_t* __restrict__ keys_out,
-                                                          uint32_t* __restrict__ vals_out,
-                                                          const uint32_t* __restrict__ n_dev, int passes) {
-    constexpr int RADIX = 1 << BITS;
-    constexpr int WAVES = THREADS / 64;
-    constexpr int DPT = RADIX / THREADS;
-    constexpr int ITEMS = (int)kSmallSort / THREADS;
-    static_assert(DPT >= 1 && ITEMS >= 1 && ITEMS * THREADS == (int)kSmallSort, "block shape");
-    __shared__ uint32_t s_wave[WAVES][RADIX];
-    __shared__ uint32_t s_local[RADIX];
-    __shared__ uint32_t s_keys[kSmallSort];
-    __shared__ uint32_t s_vals[kSmallSort];
-    __shared__ uint32_t s_tmp[WAVES];
+// Stable sort of at most kRankSort changed (key, value) pairs (count read on
+// the device) by counting: the entry at compacted position p (the compaction
+// keeps index order, so position order is value order) goes to rank
+// #{q : (key_q, q) < (key_p, p)}.  Each block copies the changed keys to LDS
+// and ranks kRankKeys entries; every entry's comparisons are split into
+// kRankWaves * kRankSplit equal shares of q (lane l of wave w takes entry
+// l % kRankKeys and share w * kRankSplit + l / kRankKeys, reading four keys per
+// 16-byte LDS load that its lanes share), and the first wave adds the shares
+// up and scatters.  A larger count leaves at once (the multi-tile radix passes
+// sort it).  The work is count^2 comparisons over count / kRankKeys blocks.
+constexpr uint32_t kRankSort = 16384;
+constexpr int kRankWaves = 16;
+constexpr uint32_t kRankSplit = 4;  // 16 entries per block (2 and 1 measured slower)
+constexpr uint32_t kRankKeys = 64u / kRankSplit;
+static_assert(kRankKeys * kRankSplit == 64u, "lanes split evenly over the entries");
+__global__ __launch_bounds__(kRankWaves * 64) void k_resort_rank(const uint32_t* __restrict__ keys_in,
+                                                                 const uint32_t* __restrict__ vals_in,
+                                                                 uint32_t* __restrict__ keys_out,
+                                                                 uint32_t* __restrict__ vals_out,
+                                                                 const uint32_t* __restrict__ n_dev) {
+    __shared__ uint4 s_keys4[kRankSort / 4u];
+    __shared__ uint32_t s_rank[kRankWaves][64];
     const uint32_t n = *n_dev;
-    if (n == 0 || n > kSmallSort) return;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (uint32_t i = (uint32_t)tid; i < n; i += THREADS) {
-        s_keys[i] = keys_in[i];
-        s_vals[i] = vals_in[i];
+    const uint32_t first = blockIdx.x * kRankKeys;
+    if (n > kRankSort || first >= n) return;
+    uint32_t* s_keys = reinterpret_cast<uint32_t*>(s_keys4);
+    for (uint32_t i = threadIdx.x; i < n; i += kRankWaves * 64u) s_keys[i] = keys_in[i];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t p = first + lane % kRankKeys;
+    const uint32_t share = w * kRankSplit + lane / kRankKeys;
+    const uint32_t kp = p < n ? s_keys[p] : 0xFFFFFFFFu;
+    const uint64_t mine = ((uint64_t)kp << 32) | p;
+    // shares of a multiple of four keys, so that every 16-byte read is aligned
+    const uint32_t chunk = ((n + kRankWaves * kRankSplit - 1u) / (kRankWaves * kRankSplit) + 3u) & ~3u;
+    const uint32_t q0 = min(n, share * chunk), q1 = min(n, q0 + chunk);
+    uint32_t rank = 0;
+    uint32_t q = q0;
+    for (; q + 4u <= q1; q += 4u) {
+        const uint4 k4 = s_keys4[q >> 2];
+        rank += ((((uint64_t)k4.x << 32) | q) < mine) ? 1u : 0u;
+        rank += ((((uint64_t)k4.y << 32) | (q + 1u)) < mine) ? 1u : 0u;
+        rank += ((((uint64_t)k4.z << 32) | (q + 2u)) < mine) ? 1u : 0u;
+        rank += ((((uint64_t)k4.w << 32) | (q + 3u)) < mine) ? 1u : 0u;
     }
-    const uint64_t lt = lanemask_lt();
-    const uint32_t wave_base = (uint32_t)w * 64u * ITEMS;
-    for (int pass = 0; pass < passes; pass++) {
-        const int shift = pass * BITS;
-        for (int i = tid; i < WAVES * RADIX; i += THREADS) (&s_wave[0][0])[i] = 0;
-        __syncthreads();  // the clear (and, before the first pass, the loads) are done
-        uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
+    for (; q < q1; q++) rank += ((((uint64_t)s_keys[q] << 32) | q) < mine) ? 1u : 0u;
+    s_rank[w][lane] = rank;
+    __syncthreads();
+    if (w == 0 && lane < kRankKeys && p < n) {
+        uint32_t r = 0;
 #pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            const uint32_t idx = wave_base + (uint32_t)it * 64u + (uint32_t)lane;
-            const bool valid = idx < n;
-            key[it] = valid ? s_keys[idx] : 0xFFFFFFFFu;
-            val[it] = valid ? s_vals[idx] : 0u;
-        }
+        for (int i = 0; i < kRankWaves; i++)
 #pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            const uint32_t idx = wave_base + (uint32_t)it * 64u + (uint32_t)lane;
-            const bool valid = idx < n;
-            const uint32_t d = (key[it] >> shift) & (RADIX - 1);
-            uint64_t peers = __ballot(valid);
-#pragma unroll
-            for (int b = 0; b < BITS; b++) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t m = __ballot(bit);
-                peers &= bit ? m : ~m;
-            }
-            if (valid) {
-                const uint32_t before = (uint32_t)__popcll(peers & lt);
-                const uint32_t prior = s_wave[w][d];
-                rank[it] = prior + before;
-                if (before == 0) s_wave[w][d] = prior + (uint32_t)__popcll(peers);
-            }
-        }
-        __syncthreads();
-        // per digit: exclusive offsets across waves, then the digit starts
-        uint32_t tcount[DPT];
-        uint32_t tsum = 0;
-#pragma unroll
-        for (int dd = 0; dd < DPT; dd++) {
-            const int d = tid * DPT + dd;
-            uint32_t c = 0;
-#pragma unroll
-            for (int ww = 0; ww < WAVES; ww++) {
-                const uint32_t x = s_wave[ww][d];
-                s_wave[ww][d] = c;
-                c += x;
-            }
-            tcount[dd] = c;
-            tsum += c;
-        }
-        uint32_t tot;
-        uint32_t lrun = block_exclusive_scan<WAVES>(tsum, s_tmp, tot);
-#pragma unroll
-        for (int dd = 0; dd < DPT; dd++) {
-            s_local[tid * DPT + dd] = lrun;
-            lrun += tcount[dd];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int it = 0; it < ITEMS; it++) {
-            const uint32_t idx = wave_base + (uint32_t)it * 64u + (uint32_t)lane;
-            if (idx < n) {
-                const uint32_t d = (key[it] >> shift) & (RADIX - 1);
-                const uint32_t pos = s_local[d] + s_wave[w][d] + rank[it];
-                s_keys[pos] = key[it];
-                s_vals[pos] = val[it];
-            }
-        }
-        __syncthreads();  // the scatter is done before the next pass clears s_wave and reads s_keys
-    }
-    for (uint32_t i = (uint32_t)tid; i < n; i += THREADS) {
-        keys_out[i] = s_keys[i];
-        vals_out[i] = s_vals[i];
+            for (uint32_t h = 0; h < kRankSplit; h++) r += s_rank[i][h * kRankKeys + lane];
+        keys_out[r] = kp;
+        vals_out[r] = vals_in[p];
     }
 }
 
@@ -585,10 +536,20 @@ __global__ __launch_bounds__(256) void k_resort_count(const uint32_t* __restrict
     if (blockIdx.x == 0)
         for (uint32_t i = threadIdx.x; i < clear_words; i += kThreads) clear[i] = 0u;
     const uint32_t base = blockIdx.x * (uint32_t)TILE;
+    // every load of the thread is issued before the first compare (indices
+    // past n read the last entry and count nothing)
+    uint32_t a[ITEMS], b[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const uint32_t idx = min(base + (uint32_t)(j * kThreads + (int)threadIdx.x), n - 1u);
+        a[j] = keys[idx];
+        b[j] = old_keys[idx];
+    }
     uint32_t c = 0;
-    for (int i = threadIdx.x; i < TILE; i += kThreads) {
-        const uint32_t idx = base + (uint32_t)i;
-        c += (idx < n && keys[idx] != old_keys[idx]) ? 1u : 0u;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const uint32_t idx = base + (uint32_t)(j * kThreads + (int)threadIdx.x);
+        c += (idx < n && a[j] != b[j]) ? 1u : 0u;
     }
     uint32_t total;
     (void)block_exclusive_scan(c, s_tmp, total);
@@ -622,16 +583,30 @@ __global__ __launch_bounds__(256) void k_resort_compact(const uint32_t* __restri
     const int tid = threadIdx.x;
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)TILE;
-    // the tile's prefix: every block sums the counts of the tiles before it
-    // (tiles^2 / 2 reads of an L2-resident array: 2M at 8M particles; at the
-    // 2^28-particle limit about 0.3% of a substep)
-    uint32_t before = 0;
-    for (uint32_t t = (uint32_t)tid; t < tile; t += kThreads) before += tile_counts[t];
     uint32_t prefix;
-    (void)block_exclusive_scan(before, s_tmp, prefix);
-    for (int i = tid; i < TILE; i += kThreads) {
-        const uint32_t idx = base + (uint32_t)i;
-        s_flag[i] = (idx < n && keys[idx] != old_keys[idx]) ? 1 : 0;
+    // the thread's keys (entries tid + j * kThreads of the tile), every load
+    // issued before the first compare
+    uint32_t kk[ITEMS];
+    {
+        uint32_t ko[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const uint32_t idx = min(base + (uint32_t)(j * kThreads + tid), n - 1u);
+            kk[j] = keys[idx];
+            ko[j] = old_keys[idx];
+        }
+        // the tile's prefix while those loads are in flight: every block sums
+        // the counts of the tiles before it (tiles^2 / 2 reads of an
+        // L2-resident array: 2M at 8M particles; at the 2^28-particle limit
+        // about 0.3% of a substep)
+        uint32_t before = 0;
+        for (uint32_t t = (uint32_t)tid; t < tile; t += kThreads) before += tile_counts[t];
+        (void)block_exclusive_scan(before, s_tmp, prefix);
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const uint32_t idx = base + (uint32_t)(j * kThreads + tid);
+            s_flag[j * kThreads + tid] = (idx < n && kk[j] != ko[j]) ? 1 : 0;
+        }
     }
     __syncthreads();
     uint32_t local[ITEMS];
@@ -649,13 +624,15 @@ __global__ __launch_bounds__(256) void k_resort_compact(const uint32_t* __restri
         run += local[j];
     }
     __syncthreads();
-    for (int i = tid; i < TILE; i += kThreads) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int i = j * kThreads + tid;
         const uint32_t idx = base + (uint32_t)i;
         if (idx >= n) continue;
         const uint32_t e = s_data[i];
         nb[idx] = e;
         if (s_flag[i]) {
-            const uint32_t k = keys[idx];
+            const uint32_t k = kk[j];
             bkeys[e] = k;
             bvals[e] = idx;
             for (int ps = 0; ps < passes; ps++) atomicAdd(&s_hist[ps * radix + ((k >> (ps * bits)) & mask)], 1u);
@@ -988,15 +965,15 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
                        workspace + p.off_status, (uint32_t)(kThreads * p.items), (size_t)p.num_tiles * rs_radix);
     VFS_TRY(check_launch("sort_scan"));
     if (timer) timer->end(s);
-    // a few changed keys are sorted by one block in LDS, more by the radix passes
+    // up to kRankSort changed keys are ranked by counting, more by the radix passes
     static_assert(kMaxPasses * 1024 >= 3 * 1024, "histograms of the 10-bit re-sort digits fit");
     if (timer) timer->begin(s, "sort_small");
-    hipLaunchKernelGGL((k_resort_small<10, kSmallThreads>), dim3(1), dim3(kSmallThreads), 0, s, bkeys, bvals, bks,
-                       bis, state, p.rs_passes);
+    hipLaunchKernelGGL(k_resort_rank, dim3((std::min(n, kRankSort) + kRankKeys - 1u) / kRankKeys), dim3(kRankWaves * 64), 0, s, bkeys,
+                       bvals, bks, bis, state);
     VFS_TRY(check_launch("sort_small"));
     if (timer) timer->end(s);
     VFS_TRY(radix_sort(p, p.rs_bits, p.rs_passes, bkeys, bvals, bks, bis, state, workspace, s, timer, true,
-                       kSmallSort));
+                       kRankSort));
     if (timer) timer->begin(s, "sort_merge");
     if (move)
         hipLaunchKernelGGL(k_resort_merge<true>, dim3(grid_for(n)), dim3(256), 0, s, keys_in, nb, n, state, bks,
