@@ -74,7 +74,9 @@ void free_adjoint(vfs_adjoint* a) {
 }
 
 // One forward substep writing its sorted state, densities, boundary terms,
-// permutation and neighbour lists into a tape slot.
+// permutation and neighbour lists into a tape slot.  It sorts as the
+// simulation's own substeps do: from scratch the first time, then by
+// re-sorting the keys that changed bucket.
 int record_taped_substep(vfs_sim* sim, const vfs_adjoint::Slot& slot, hipStream_t s) {
     const uint32_t n = sim->n;
     StepBuffers b = sim->buffers();
@@ -85,10 +87,21 @@ int record_taped_substep(vfs_sim* sim, const vfs_adjoint::Slot& slot, hipStream_
     b.bnd = slot.bnd;
     b.nbr = slot.nbr;
     b.nbr_count = slot.cnt;
-    VFS_TRY(sort_keys_iota(sim->plan, sim->keys, sim->sorted_keys, slot.perm, sim->sort_ws, s, nullptr,
-                           sim->offsets, (size_t)n + 1, n));
-    VFS_TRY(launch_reorder_state(b, s));
-    VFS_TRY(launch_cell_start_scan(sim->plan, sim->offsets, sim->cell_start, sim->sort_ws, s));
+    if (sim->resort_ready) {
+        // re-sort against the last sort's keys and bucket starts (a consistent
+        // pair whatever state was restored since); the merge moves x and v
+        // into the slot's sorted arrays
+        VFS_TRY(resort_keys_iota(sim->plan, sim->keys, sim->sorted_keys, slot.perm, sim->cell_start, sim->sort_ws, s,
+                                 nullptr, &b));
+        sim->last_full_sort = false;
+    } else {
+        VFS_TRY(sort_keys_iota(sim->plan, sim->keys, sim->sorted_keys, slot.perm, sim->sort_ws, s, nullptr,
+                               sim->offsets, (size_t)n + 1, n));
+        VFS_TRY(launch_reorder_state(b, s));
+        VFS_TRY(launch_cell_start_scan(sim->plan, sim->offsets, sim->cell_start, sim->sort_ws, s));
+        sim->resort_ready = true;
+        sim->last_full_sort = true;
+    }
     const SimConsts& c = sim->scenes[0].consts;
     VFS_TRY(launch_density(b, c, nullptr, sim->objs_dev, sim->tuning.block_size, sim->tuning.stage_rows, s));
     VFS_TRY(launch_forces_integrate(b, c, nullptr, sim->tuning.block_size, s));
