@@ -150,3 +150,43 @@ def test_torch_autograd_function_wrapper():
     r = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=300)
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stderr[-4000:]
+
+
+def test_backward_rejects_lists_past_the_capacity():
+    """The reverse pass differentiates complete neighbour lists only: when a
+    recomputed substep's list outgrows VFS_NEIGHBOUR_CAPACITY (particles at
+    0.3 h spacing, ~150 neighbours each) the backward stops with a state error
+    naming the capacity instead of returning gradients of truncated lists.
+    The check is the maximum list length over every particle of every
+    substep of a segment, so it also pins that reduction."""
+    h = 0.05
+    x = CF.lattice((10, 10, 10), 0.3 * h, (0.3, 0.3, 0.3), 0.03 * h, 5)
+    n = x.shape[0]
+    p = V.make_params(n, h, dt=1e-5, box_size=(1.0, 1.0, 1.0))
+    sim = V.Sim(p, [])
+    adj = V.Adjoint(sim, 4, 2, (0.0, 0.0, 0.0), (0.1, 0.1, 0.1), (8, 8, 8))
+    adj.forward(x, np.zeros_like(x))
+    _, mx, over = sim.neighbour_stats()
+    assert mx > V.VFS_NEIGHBOUR_CAPACITY and over > 0
+    with pytest.raises(V.VfsError, match="VFS_NEIGHBOUR_CAPACITY"):
+        adj.backward()
+    adj.close()
+    sim.close()
+
+
+def test_backward_accepts_lists_at_the_capacity_edge():
+    """The same check passes when the longest list fits: a lattice at 0.5 h
+    spacing (~33 neighbours) runs the backward to the end."""
+    h = 0.05
+    x = CF.lattice((12, 12, 12), 0.5 * h, (0.3, 0.3, 0.3), 0.03 * h, 6)
+    n = x.shape[0]
+    p = V.make_params(n, h, dt=1e-5, box_size=(1.0, 1.0, 1.0))
+    sim = V.Sim(p, [])
+    adj = V.Adjoint(sim, 4, 2, (0.0, 0.0, 0.0), (0.1, 0.1, 0.1), (8, 8, 8))
+    adj.forward(x, np.zeros_like(x))
+    _, mx, _ = sim.neighbour_stats()
+    assert 0 < mx <= V.VFS_NEIGHBOUR_CAPACITY
+    gx, gv, gnu = adj.backward()
+    assert np.all(np.isfinite(gx)) and np.all(np.isfinite(gv)) and np.isfinite(gnu)
+    adj.close()
+    sim.close()

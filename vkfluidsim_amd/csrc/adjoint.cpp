@@ -173,7 +173,7 @@ int vfs_adjoint_create(vfs_sim* sim, uint32_t n_substeps, uint32_t checkpoint_ev
     r |= dev_alloc(&a->dup, n);
     r |= dev_alloc(&a->maxc, 1);
     r |= dev_alloc(&a->nu_bar, 1);
-    r |= dev_alloc(&a->nu_part, ((size_t)n + 255) / 256);
+    r |= dev_alloc(&a->nu_part, ((size_t)n * 4 + 255) / 256);  // one per block of the four-lane force pass
     r |= dev_alloc(&a->rho_grid, G);
     r |= dev_alloc(&a->target, G);
     if (r) {

@@ -391,22 +391,37 @@ __global__ __launch_bounds__(256) void k_adj_integrate(TapeView t, AdjBuffers a,
 // terms), of the EOS terms and so of rho, of grad psi, and of the viscosity.
 // The pair terms are evaluated in double from the float tape (positions,
 // velocities, densities): the per-particle sums cancel strongly.
+// Four lanes per particle in the two neighbour kernels: lane `sub` of a
+// particle takes entries sub, sub + 4, ... of its list (entry sub of every
+// chunk, so the four lanes read a chunk as one 16-byte run), and the partial
+// sums are added over the four lanes (xor 1, then xor 2: every lane ends with
+// the same total).  A substep of 65,536 particles is otherwise one wave per
+// SIMD, each walking its lists alone.
+constexpr uint32_t kAdjLanes = 4;
+
+__device__ __forceinline__ double quad_sum(double v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    return v;
+}
+
 __global__ __launch_bounds__(256) void k_adj_forces(TapeView t, AdjBuffers a, SimConsts c, AdjConsts k) {
     __shared__ double s_nu[256 / 64];
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t gtid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t i = gtid / kAdjLanes, sub = gtid % kAdjLanes;
     double nu_local = 0.0;
-    if (i < a.n) {
+    double Pbar = 0.0, Mbar = 0.0, gx[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
+    double fv[3] = {0.0, 0.0, 0.0};
+    if (i < a.n) {  // the four lanes of a particle are active together
         const float4 xi = t.xs[i], vi = t.vs[i];
         const float rhoi = t.rho[i];
         const double Pi = pd_d(rhoi, k), Mi = md_d(rhoi, k);
         const double4 abi = a.abar[i];
         const bool dup_i = a.dup[i] != 0u;
         const double ai[3] = {abi.x, abi.y, abi.z};
-        double Pbar = 0.0, Mbar = 0.0, gx[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0};
-        double fv[3] = {0.0, 0.0, 0.0};
         const uint32_t count = t.cnt[i];
         const uint32_t* list = t.nbr;
-        for (uint32_t e = 0; e < count; e++) {
+        for (uint32_t e = sub; e < count; e += kAdjLanes) {
             const uint32_t j = list[(size_t)(e >> 2) * 4u * a.n + 4u * i + (e & 3u)];
             if (j == i) continue;
             const float4 xj = t.xs[j], vj = t.vs[j];
@@ -453,6 +468,20 @@ __global__ __launch_bounds__(256) void k_adj_forces(TapeView t, AdjBuffers a, Si
                 gv[q] += sb * xij[q];
             }
         }
+        Pbar = quad_sum(Pbar);
+        Mbar = quad_sum(Mbar);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            gx[q] = quad_sum(gx[q]);
+            gv[q] = quad_sum(gv[q]);
+            fv[q] = quad_sum(fv[q]);
+        }
+    }
+    if (i < a.n && sub == 0u) {
+        const float rhoi = t.rho[i];
+        const double Pi = pd_d(rhoi, k);
+        const double4 abi = a.abar[i];
+        const double ai[3] = {abi.x, abi.y, abi.z};
         // viscosity: a_i = ... + nu10 * fv, nu10 = 10 nu
         nu_local = 10.0 * (ai[0] * fv[0] + ai[1] * fv[1] + ai[2] * fv[2]);
         // boundary pressure: a_i = ... - (Pi + pb) rho0 grad psi
@@ -501,19 +530,36 @@ __global__ __launch_bounds__(256) void k_add_partials(const double* __restrict__
     if (threadIdx.x == 0) *total += s[0];
 }
 
-// Density and boundary volume backwards, then the adjoints of the substep's
-// input in the order of its input arrays (inverse of the reorder).
-__global__ __launch_bounds__(256) void k_adj_density(TapeView t, AdjBuffers a, SimConsts c, AdjConsts k,
-                                                     const DevObject* __restrict__ objs) {
+// Boundary volume backwards (particles with boundary objects only): adds the
+// adjoint of psi and grad psi through the grids' trilinear gradients into the
+// position adjoint, before the density pass reads it.  A kernel of its own:
+// its double grid arithmetic needs far more registers than the neighbour sums.
+__global__ __launch_bounds__(256) void k_adj_boundary(TapeView t, AdjBuffers a, SimConsts c, AdjConsts k,
+                                                      const DevObject* __restrict__ objs) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= a.n) return;
     const float4 xi = t.xs[i];
+    const double4 g0 = a.gx[i], gp = a.gpsi[i];
+    double gx[3] = {g0.x, g0.y, g0.z};
+    const double gb[3] = {gp.x, gp.y, gp.z};
+    const float pos[3] = {xi.x, xi.y, xi.z};
+    boundary_volume_adjoint(objs + c.obj_begin, c.obj_count, pos, k, k.rho0 * a.rbar[i], gb, gx);
+    a.gx[i] = make_double4(gx[0], gx[1], gx[2], 0.0);
+}
+
+// Density backwards (four lanes per particle, as k_adj_forces), then the
+// adjoints of the substep's input in the order of its input arrays (inverse
+// of the reorder).
+__global__ __launch_bounds__(256) void k_adj_density(TapeView t, AdjBuffers a, SimConsts c, AdjConsts k) {
+    const uint32_t gtid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t i = gtid / kAdjLanes, sub = gtid % kAdjLanes;
+    if (i >= a.n) return;  // the four lanes of a particle leave together
+    const float4 xi = t.xs[i];
     const double rbi = a.rbar[i];
     const bool dup_i = a.dup[i] != 0u;
-    const double4 g0 = a.gx[i];
-    double gx[3] = {g0.x, g0.y, g0.z};
+    double gx[3] = {0.0, 0.0, 0.0};
     const uint32_t count = t.cnt[i];
-    for (uint32_t e = 0; e < count; e++) {
+    for (uint32_t e = sub; e < count; e += kAdjLanes) {
         const uint32_t j = t.nbr[(size_t)(e >> 2) * 4u * a.n + 4u * i + (e & 3u)];
         if (j == i) continue;
         const float4 xj = t.xs[j];
@@ -525,14 +571,12 @@ __global__ __launch_bounds__(256) void k_adj_density(TapeView t, AdjBuffers a, S
 #pragma unroll
         for (int q = 0; q < 3; q++) gx[q] += s * xij[q];
     }
-    if (c.obj_count) {
-        const double4 gp = a.gpsi[i];
-        const double gb[3] = {gp.x, gp.y, gp.z};
-        const float pos[3] = {xi.x, xi.y, xi.z};
-        boundary_volume_adjoint(objs + c.obj_begin, c.obj_count, pos, k, k.rho0 * rbi, gb, gx);
-    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) gx[q] = quad_sum(gx[q]);
+    if (sub != 0u) return;
+    const double4 g0 = a.gx[i];
     const uint32_t o = t.perm[i];
-    a.gx_out[o] = make_double4(gx[0], gx[1], gx[2], 0.0);
+    a.gx_out[o] = make_double4(g0.x + gx[0], g0.y + gx[1], g0.z + gx[2], 0.0);
     a.gv_out[o] = a.gv[i];
 }
 
@@ -653,12 +697,31 @@ __global__ __launch_bounds__(256) void k_grid_loss_grad(GridDesc gd, const float
     gx[i] = make_double4(acc[0], acc[1], acc[2], 0.0);
 }
 
-__global__ void k_max_u32(const uint32_t* __restrict__ a, uint32_t n, uint32_t* __restrict__ out) {
+// out = max(out, max of a[0..n)): sixteen elements per thread (loads issued
+// together), a block reduction, and one atomic per block (atomics on one
+// address serialise, so a thousand of them per launch cost ~10 us).
+constexpr uint32_t kMaxPerThread = 16;
+__global__ __launch_bounds__(256) void k_max_u32(const uint32_t* __restrict__ a, uint32_t n,
+                                                 uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_max[256 / 64];
+    const uint32_t base = blockIdx.x * 256u * kMaxPerThread + threadIdx.x;
+    uint32_t v[kMaxPerThread];
+#pragma unroll
+    for (uint32_t u = 0; u < kMaxPerThread; u++) {
+        const uint32_t i = base + u * 256u;
+        v[u] = i < n ? a[i] : 0u;
+    }
     uint32_t m = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = max(m, a[i]);
+#pragma unroll
+    for (uint32_t u = 0; u < kMaxPerThread; u++) m = max(m, v[u]);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63u) == 0u) atomicMax(out, m);
+    if ((threadIdx.x & 63u) == 0u) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 256 / 64; w++) m = max(m, s_max[w]);
+        atomicMax(out, m);
+    }
 }
 
 __global__ void k_f3_to_f4(const float* __restrict__ src, float4* __restrict__ dst, uint32_t n) {
@@ -754,14 +817,19 @@ int launch_adjoint_substep(const uint32_t* perm, const float4* xs, const float4*
     const TapeView t{perm, xs, vs, rho, bnd, nbr, cnt};
     const AdjBuffers a{gx_in, gv_in, abar, gx, gv, rbar, gpsi, dup, gx_out, gv_out, nu_part, n};
     const uint32_t nb = blocks_for(n, 256);
+    const uint32_t nb_lanes = blocks_for((uint64_t)n * kAdjLanes, 256);  // four lanes per particle
     const AdjConsts k = adj_consts(c);
     hipLaunchKernelGGL(k_adj_integrate, dim3(nb), dim3(256), 0, s, t, a, c, k);
     VFS_TRY(check_launch("adj_integrate"));
-    hipLaunchKernelGGL(k_adj_forces, dim3(nb), dim3(256), 0, s, t, a, c, k);
+    hipLaunchKernelGGL(k_adj_forces, dim3(nb_lanes), dim3(256), 0, s, t, a, c, k);
     VFS_TRY(check_launch("adj_forces"));
-    hipLaunchKernelGGL(k_add_partials, dim3(1), dim3(256), 0, s, nu_part, nb, nu_bar);
+    hipLaunchKernelGGL(k_add_partials, dim3(1), dim3(256), 0, s, nu_part, nb_lanes, nu_bar);
     VFS_TRY(check_launch("add_partials"));
-    hipLaunchKernelGGL(k_adj_density, dim3(nb), dim3(256), 0, s, t, a, c, k, objs);
+    if (c.obj_count) {
+        hipLaunchKernelGGL(k_adj_boundary, dim3(nb), dim3(256), 0, s, t, a, c, k, objs);
+        VFS_TRY(check_launch("adj_boundary"));
+    }
+    hipLaunchKernelGGL(k_adj_density, dim3(nb_lanes), dim3(256), 0, s, t, a, c, k);
     return check_launch("adj_density");
 }
 
@@ -787,7 +855,7 @@ int launch_grid_loss_grad(const float* origin, const float* spacing, const uint3
 
 int launch_max_u32(const uint32_t* a, uint32_t n, uint32_t* out, hipStream_t s) {
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_max_u32, dim3(std::min<uint32_t>(blocks_for(n, 256), 1024u)), dim3(256), 0, s, a, n, out);
+    hipLaunchKernelGGL(k_max_u32, dim3(blocks_for(n, 256u * kMaxPerThread)), dim3(256), 0, s, a, n, out);
     return check_launch("max_u32");
 }
 
