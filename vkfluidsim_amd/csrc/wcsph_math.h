@@ -228,12 +228,24 @@ __device__ void boundary_volume(const DevObject* __restrict__ objs, uint32_t n_o
 // ------------------------------------------------------------------ kernels
 
 // The hardware deals workgroups to the 8 XCDs round robin; each XCD has its own
-// L2.  Giving XCD x the contiguous block range [x * nb/8, (x+1) * nb/8) keeps a
-// compact slab of sorted particles (and its neighbour cells) in one L2 instead
-// of every L2 streaming the whole particle set.  nblocks is a multiple of 8.
+// L2.  XCD x is given chunks x, x + 8, x + 16, ... of kXcdChunk consecutive
+// blocks (compact slabs of sorted particles, so a chunk's neighbour cells are
+// mostly in the same L2), and the blocks past the last whole round of chunks
+// contiguously.  Chunks rather than one contiguous eighth per XCD: where the
+// fluid is uneven an eighth of the sorted order can hold 7% more neighbour work
+// than the mean, and the slowest XCD sets the kernel's end.  Measured on 1M
+// particles in contact: 16 blocks per chunk 0.452 ms per substep, 8 0.456,
+// 64 0.455, 256 0.458, one eighth per XCD 0.459, no remap (1) 0.472.
+// nblocks is a multiple of 8.
+constexpr uint32_t kXcdChunk = 16;
+static_assert(kXcdChunk >= 1u && kXcdChunk <= (1u << 28), "8 chunks must fit the 32-bit block count");
 __device__ __forceinline__ uint32_t xcd_block(uint32_t nblocks) {
-    const uint32_t b = blockIdx.x;
-    return (b & 7u) * (nblocks >> 3) + (b >> 3);
+    const uint32_t b = blockIdx.x, x = b & 7u, k = b >> 3;
+    const uint32_t round = 8u * kXcdChunk;
+    const uint32_t full = nblocks / round;  // whole rounds of eight chunks
+    if (k < full * kXcdChunk) return (k / kXcdChunk) * round + x * kXcdChunk + k % kXcdChunk;
+    const uint32_t rest = (nblocks - full * round) >> 3;
+    return full * round + x * rest + (k - full * kXcdChunk);
 }
 
 // Raw buffer resources over the particle arrays: 32-bit offsets and hardware
