@@ -472,6 +472,42 @@ def test_resort_sorts_changed_keys_by_rank_up_to_16384(pairs):
     sim.close()
 
 
+@pytest.mark.parametrize("moved", [3000, 16000])
+def test_resort_rank_with_many_equal_changed_keys(moved):
+    """Stability of the ranking by counting under ties: after a first
+    substep, `moved` resting particles (in index order scattered over the
+    volume) are placed inside one common cell, so that almost every changed
+    key is the same key and the sorted order inside that bucket is decided
+    by the index alone.  Hash tables and state match the oracle bit for bit
+    after the re-sort (3,000: ranked by counting; 16,000: still at most
+    16,384 changed keys)."""
+    rng = np.random.default_rng(moved)
+    n = 40_000
+    h = 0.03
+    xs = rng.uniform(0.05, 0.95, size=(n, 3)).astype(np.float32)
+    vs = np.zeros((n, 3), np.float32)
+    p = V.make_params(n, h, dt=1e-6, box_size=(1.0, 1.0, 1.0), gravity=(0.0, 0.0, 0.0), viscosity=0.0,
+                      stiffness=0.0, flags=V.VFS_FLAG_TRACK_IDS)
+    sim = V.Sim(p, [])
+    orc = H.OracleSim(p, [])
+    for s in (sim, orc):
+        s.set_state(xs, vs)
+        s.substeps(1)
+    x, v, _, _ = sim.get_state()
+    pick = np.sort(rng.permutation(n)[:moved])
+    # inside the cell [0.5, 0.5 + h)^3 (away from its faces), distinct positions
+    x = x.copy()
+    x[pick] = (0.5 + h * rng.uniform(0.1, 0.9, size=(moved, 3))).astype(np.float32)
+    for s in (sim, orc):
+        s.set_state(x, v)
+        s.substeps(1)
+    changed = sim.changed_keys()
+    print(f"case {moved} moved into one cell: {changed} changed keys", flush=True)
+    assert moved * 0.9 <= changed <= 16384
+    _assert_same_step(sim, orc)
+    sim.close()
+
+
 def test_resort_every_key_changed_at_300k():
     """The multi-tile re-sort with nearly every key changed: 300,001 particles
     (a ragged count) shuffled after their first substeps, so that almost every
