@@ -4,13 +4,14 @@ ascending previous key array `old`, its bucket starts `cs` (first index with
 old >= k) and any new keys, the changed entries (key differs from old at the
 same index) are ranked among themselves and every entry gets
 
-    unchanged j:      (j - nb[j]) + #{changed b : (key_b, b) < (key_j, j)}
+    unchanged j:      (j - nb[j]) + lb[key_j] + #{changed b : key_b = key_j, b < j}
     changed, rank r:  (c - nb[c]) + r,  c = clamp(b, cs[key_b], cs[key_b + 1])
 
-(nb = changed entries before an index), which must equal the position of the
-entry in np.argsort(keys, kind="stable"); the updated bucket starts
-(cs[k] - nb[cs[k]] + #{changed keys < k}) must equal the bucket starts of the
-new keys.  A dropped term, a swapped clamp bound or an unstable tie breaks
+(nb = changed entries before an index, lb[k] = changed keys below k; the
+changed entries of key k are ranks lb[k] .. lb[k + 1] - 1, in index order),
+which must equal the position of the entry in np.argsort(keys,
+kind="stable"); the updated bucket starts (cs[k] - nb[cs[k]] + lb[k]) must
+equal the bucket starts of the new keys.  A dropped term, a swapped clamp bound or an unstable tie breaks
 one of the cases (few, all, none changed; duplicates; a table of one key)."""
 import numpy as np
 import pytest
@@ -28,12 +29,12 @@ def _resort(old, keys, table):
     r = np.arange(len(bis))
     c = np.clip(bis, cs[bks], cs[bks + 1])
     out[bis] = (c - nb[c]) + r
+    lb = np.searchsorted(bks, np.arange(table + 1), side="left")
     u = np.flatnonzero(~changed)
-    # lower bound of (key_j, j) among the sorted changed pairs
-    composite_b = bks.astype(np.int64) * (n + 1) + bis
-    composite_u = keys[u].astype(np.int64) * (n + 1) + u
-    out[u] = (u - nb[u]) + np.searchsorted(composite_b, composite_u, side="left")
-    new_cs = (cs - nb[cs]) + np.searchsorted(bks, np.arange(table + 1), side="left")
+    lo, hi = lb[keys[u]], lb[keys[u] + 1]
+    same_key_before = np.array([np.searchsorted(bis[a:z], j) for a, z, j in zip(lo, hi, u)], np.int64)
+    out[u] = (u - nb[u]) + lo + same_key_before
+    new_cs = (cs - nb[cs]) + lb
     return out, new_cs
 
 

@@ -690,8 +690,11 @@ __device__ __forceinline__ uint32_t wave_partition_point(uint32_t lo, uint32_t h
 }
 
 // Output positions of both runs: threads below the changed count place the
-// sorted changed entry of their index, every unchanged index finds how many
-// changed entries precede it by binary search.  MOVE: each placed entry also
+// sorted changed entry of their index (previous bucket starts in old_cs), every
+// unchanged index adds the changed entries that precede it: lb[key] of them
+// have a smaller key (written by k_resort_cell_starts), and those with the
+// same key and a lower index sit at the front of [lb[key], lb[key + 1]) (a
+// search only when that range is not empty).  MOVE: each placed entry also
 // moves its particle's rows to the sorted place (Reorder, reorder.slang:11-19,
 // as a scatter), x and v (and ids, predictions when present) from mv's
 // current-order arrays into its sorted ones.
@@ -701,7 +704,8 @@ __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ state,
                                                       const uint32_t* __restrict__ bks,
                                                       const uint32_t* __restrict__ bis,
-                                                      const uint32_t* __restrict__ cell_start,
+                                                      const uint32_t* __restrict__ old_cs,
+                                                      const uint32_t* __restrict__ lb,
                                                       uint32_t* __restrict__ keys_out,
                                                       uint32_t* __restrict__ perm_out, StepBuffers mv) {
     const uint32_t g = blockIdx.x * (uint32_t)kThreads + threadIdx.x;
@@ -709,7 +713,7 @@ __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict
     const uint32_t count = state[0];
     if (g < count) {
         const uint32_t kb = bks[g], ib = bis[g];
-        const uint32_t lo = cell_start[kb], hi = cell_start[kb + 1u];
+        const uint32_t lo = old_cs[kb], hi = old_cs[kb + 1u];
         const uint32_t c = min(max(ib, lo), hi);
         const uint32_t pos = (c - nb[c]) + g;
         if (pos < n) {  // always, for an ascending previous key array and its bucket starts
@@ -735,11 +739,10 @@ __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict
         if (mv.pid) ig = mv.pid[g];
         if (mv.ps) pg = mv.pred[g];
     }
-    uint32_t lo = 0, hi = count;
+    uint32_t lo = lb[kj], hi = lb[kj + 1u];  // kj < n: lb has n + 1 entries
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        const uint32_t km = bks[mid];
-        if (km < kj || (km == kj && bis[mid] < g))
+        if (bis[mid] < g)
             lo = mid + 1u;
         else
             hi = mid;
@@ -763,18 +766,20 @@ __global__ __launch_bounds__(256) void k_resort_merge(const uint32_t* __restrict
 // lower bound in the sorted changed keys).  The lower bounds of the block's
 // keys lie between those of its first key and of the key after its last, so
 // waves 0 and 1 find those two among all changed keys (a wave-wide search)
-// and every thread searches only that window.
+// and every thread searches only that window.  The previous start goes to
+// old_cs and the lower bound to lb, for the merge that runs after it.
 // m = table size + 1 entries.
 __global__ __launch_bounds__(256) void k_resort_cell_starts(uint32_t* __restrict__ cs, const uint32_t* __restrict__ nb,
                                                             const uint32_t* __restrict__ bks,
-                                                            const uint32_t* __restrict__ state, uint32_t m) {
+                                                            const uint32_t* __restrict__ state, uint32_t m,
+                                                            uint32_t* __restrict__ old_cs, uint32_t* __restrict__ lb) {
     __shared__ uint32_t s_window[2];
     const uint32_t k0 = blockIdx.x * (uint32_t)kThreads;
     const uint32_t k = k0 + threadIdx.x;
     const uint32_t count = state[0];
-    uint32_t below = 0u;
+    uint32_t below = 0u, c = 0u;
     if (k < m) {
-        const uint32_t c = cs[k];
+        c = cs[k];
         below = c - nb[c];
     }
     const uint32_t wave = threadIdx.x >> 6;
@@ -785,7 +790,10 @@ __global__ __launch_bounds__(256) void k_resort_cell_starts(uint32_t* __restrict
     }
     __syncthreads();
     if (k >= m) return;
-    cs[k] = below + changed_lower_bound(bks, k, s_window[0], s_window[1]);
+    const uint32_t l = changed_lower_bound(bks, k, s_window[0], s_window[1]);
+    old_cs[k] = c;
+    lb[k] = l;
+    cs[k] = below + l;
 }
 
 inline uint32_t grid_for(uint64_t n, uint32_t block = 256) {
@@ -833,6 +841,8 @@ SortPlan make_sort_plan(uint32_t n, uint32_t max_value, int bits_per_pass, int i
     p.off_rs_bvals = take(rs_n);
     p.off_rs_bks = take(rs_n);
     p.off_rs_bis = take(rs_n);
+    p.off_rs_old_cs = take(resort ? rs_n + 1 : 0);
+    p.off_rs_lb = take(resort ? rs_n + 1 : 0);
     p.rs_bits = 10;
     p.rs_passes = kb == 0 ? 1 : (kb + p.rs_bits - 1) / p.rs_bits;
     const size_t rs_radix = resort ? (size_t)1 << p.rs_bits : 0;
@@ -974,19 +984,23 @@ static int sort_impl(const SortPlan& p, const uint32_t* keys_in, uint32_t* keys_
     if (timer) timer->end(s);
     VFS_TRY(radix_sort(p, p.rs_bits, p.rs_passes, bkeys, bvals, bks, bis, state, workspace, s, timer, true,
                        kRankSort));
+    // the bucket starts first: they also leave the previous starts and each
+    // key's count of smaller changed keys for the merge
+    uint32_t* old_cs = workspace + p.off_rs_old_cs;
+    uint32_t* lb = workspace + p.off_rs_lb;
+    if (timer) timer->begin(s, "cell_ranges");
+    hipLaunchKernelGGL(k_resort_cell_starts, dim3(grid_for((uint64_t)n + 1)), dim3(256), 0, s, cell_start, nb, bks,
+                       state, n + 1u, old_cs, lb);
+    VFS_TRY(check_launch("cell_ranges"));
+    if (timer) timer->end(s);
     if (timer) timer->begin(s, "sort_merge");
     if (move)
         hipLaunchKernelGGL(k_resort_merge<true>, dim3(grid_for(n)), dim3(256), 0, s, keys_in, nb, n, state, bks,
-                           bis, cell_start, keys_out, perm_out, *move);
+                           bis, old_cs, lb, keys_out, perm_out, *move);
     else
         hipLaunchKernelGGL(k_resort_merge<false>, dim3(grid_for(n)), dim3(256), 0, s, keys_in, nb, n, state, bks,
-                           bis, cell_start, keys_out, perm_out, StepBuffers{});
+                           bis, old_cs, lb, keys_out, perm_out, StepBuffers{});
     VFS_TRY(check_launch("sort_merge"));
-    if (timer) timer->end(s);
-    if (timer) timer->begin(s, "cell_ranges");
-    hipLaunchKernelGGL(k_resort_cell_starts, dim3(grid_for((uint64_t)n + 1)), dim3(256), 0, s, cell_start, nb, bks,
-                       state, n + 1u);
-    VFS_TRY(check_launch("cell_ranges"));
     if (timer) timer->end(s);
     return 0;
 }

@@ -36,6 +36,9 @@ struct SortPlan {
     int rs_bits, rs_passes;
     uint32_t rs_tiles;
     size_t off_rs_nb, off_rs_bkeys, off_rs_bvals, off_rs_bks, off_rs_bis;
+    // previous bucket starts and each key's count of smaller changed keys
+    // (n + 1 entries each), written by the bucket-start update for the merge
+    size_t off_rs_old_cs, off_rs_lb;
     size_t off_rs_tile_counts, off_rs_state;
     size_t zero_begin;
     size_t workspace_words;
