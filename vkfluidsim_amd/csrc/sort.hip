@@ -128,17 +128,19 @@ __global__ __launch_bounds__(256) void k_radix_pass(const uint32_t* __restrict__
     __shared__ uint32_t s_tmp[kWaves];
 
     // a device-side count of at most small_cap keys was sorted by k_resort_rank
-    if (n_dev && *n_dev <= small_cap) return;
+    if (n_dev) {
+        n = *n_dev;
+        if (n <= small_cap) return;
+    }
+    // Only the first ceil(n / TILE) blocks claim tiles (they get tiles 0, 1, ...
+    // in claim order); the others leave without touching the counter or LDS.
+    if (blockIdx.x >= (n + (uint32_t)TILE - 1u) / (uint32_t)TILE) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     for (int i = tid; i < kWaves * RADIX; i += kThreads) (&s_wave[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint32_t base = tile * (uint32_t)TILE;
-    // With a device-side count, tiles past it have nothing to sort; tiles are
-    // claimed in order and look back only at earlier ones, so they just leave.
-    if (n_dev) n = *n_dev;
-    if (base >= n) return;
+    const uint32_t base = tile * (uint32_t)TILE;  // < n: tiles past the count are never claimed
     const uint32_t wave_base = base + (uint32_t)w * 64u * ITEMS;
 
     uint32_t key[ITEMS], val[ITEMS], rank[ITEMS];
