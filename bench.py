@@ -427,6 +427,9 @@ def bench_adjoint(steps: int, every: int, repeats: int = 3) -> dict:
     t_f = min(r[0] for r in runs)
     t_b = min(r[1] for r in runs)
     loss, gv, gnu = runs[-1][2], runs[-1][3], runs[-1][4]
+    # one more reverse pass with an event pair around every launch group
+    # (outside the timed runs): total ms per stage over the pass
+    stages = adj.profile_backward()
     adj.close()
     sim.close()
     return {
@@ -438,6 +441,8 @@ def bench_adjoint(steps: int, every: int, repeats: int = 3) -> dict:
         "loss": loss,
         "grad_v0_norm": float(np.linalg.norm(gv)),
         "grad_viscosity": gnu,
+        "backward_stages_ms": {name: round(ms, 3) for name, ms, _ in stages},
+        "backward_stage_groups": {name: k for name, _, k in stages},
     }
 
 

@@ -284,6 +284,17 @@ int vfs_adjoint_forward(vfs_adjoint *adj, const float *x0_dev, const float *v0_d
  * viscosity parameter (viscosity_strenght).  The simulation is left at the
  * forward's final positions and velocities. */
 int vfs_adjoint_backward(vfs_adjoint *adj, float *grad_x0_dev, float *grad_v0_dev, double *grad_viscosity);
+/* Runs the reverse pass of the last forward once more (the gradients are not
+ * returned; the simulation is left as by vfs_adjoint_backward) with a HIP
+ * event pair around every launch group on the simulation stream, and reports
+ * per stage the total time in ms over the whole pass, the number of timed
+ * groups and the stage name (at most max_stages entries; any output array may
+ * be NULL): the loss gradient, the recomputed substeps' sort stages, density
+ * and forces_integrate, the list-capacity check, and the reverse kernels
+ * adj_integrate, adj_forces, adj_viscosity_sum, adj_boundary, adj_density.
+ * No counterpart in the reference (it has no adjoint). */
+int vfs_adjoint_profile_backward(vfs_adjoint *adj, float *ms, uint32_t *launches, const char **names,
+                                 uint32_t max_stages, uint32_t *n_stages);
 
 /* ---- compute layer drop-ins (reference src/compute headers) ------------ */
 

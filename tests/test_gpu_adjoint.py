@@ -73,6 +73,29 @@ def test_adjoint_forward_matches_plain_substeps():
     assert loss > 0
 
 
+def test_backward_profile_times_every_stage_and_changes_nothing():
+    """vfs_adjoint_profile_backward runs the reverse pass once more with
+    events around every launch group: the reverse kernels run once per
+    substep (20), the recomputed forward kernels once per substep, the list
+    check once per checkpoint interval (4), and the gradients of a reverse pass
+    after it are those before it, bit for bit."""
+    w, grids, p, sim, adj, target, spacing, dims = _setup()
+    adj.forward(w.x, w.v, target)
+    gx0, gv0, gnu0 = adj.backward()
+    stages = {name: (ms, k) for name, ms, k in adj.profile_backward()}
+    print(stages)
+    for name in ("adj_integrate", "adj_forces", "adj_viscosity_sum", "adj_boundary", "adj_density", "density",
+                 "forces_integrate"):
+        assert stages[name][1] == 20, name
+        assert stages[name][0] > 0.0, name
+    assert stages["list_check"][1] == 4
+    assert stages["loss_gradient"][1] == 1
+    gx1, gv1, gnu1 = adj.backward()
+    np.testing.assert_array_equal(gx0, gx1)
+    np.testing.assert_array_equal(gv0, gv1)
+    assert gnu0 == gnu1
+
+
 def _directional(adj, x0, v0, target, dx, dv):
     lp, _ = adj.forward(x0 + EPS * dx, v0 + EPS * dv, target)
     lm, _ = adj.forward(x0 - EPS * dx, v0 - EPS * dv, target)

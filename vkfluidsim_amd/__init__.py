@@ -689,6 +689,17 @@ class Adjoint:
         gv = self._gv.to_host(np.float32, 3 * self.n).reshape(self.n, 3)
         return gx, gv, gnu.value
 
+    def profile_backward(self):
+        """[(stage, total ms over the reverse pass, timed groups)] of one more
+        reverse pass of the last forward (vfs_adjoint_profile_backward)."""
+        m = 24
+        ms = (C.c_float * m)()
+        la = (C.c_uint32 * m)()
+        names = (C.c_char_p * m)()
+        cnt = C.c_uint32(0)
+        call("vfs_adjoint_profile_backward", self.handle, ms, la, names, m, C.byref(cnt))
+        return [(names[i].decode(), float(ms[i]), int(la[i])) for i in range(cnt.value)]
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             lib.vfs_adjoint_destroy(self.handle)

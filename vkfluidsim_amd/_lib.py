@@ -185,6 +185,7 @@ SIGNATURES = {
     "vfs_adjoint_destroy": (_i32, [_vp]),
     "vfs_adjoint_forward": (_i32, [_vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double)]),
     "vfs_adjoint_backward": (_i32, [_vp, _vp, _vp, C.POINTER(C.c_double)]),
+    "vfs_adjoint_profile_backward": (_i32, [_vp, _fp, _up, C.POINTER(C.c_char_p), _u32, _up]),
     "vfs_spatial_hash_create": (_i32, [_u32, C.c_float, C.POINTER(_vp)]),
     "vfs_spatial_hash_set_cell_size": (_i32, [_vp, C.c_float]),
     "vfs_spatial_hash_run": (_i32, [_vp, _vp, _u32, _vp]),

@@ -16,7 +16,8 @@ int launch_adjoint_substep(const uint32_t* perm, const float4* xs, const float4*
                            const float4* bnd, const uint32_t* nbr, const uint32_t* cnt, const double4* gx_in,
                            const double4* gv_in, double4* abar, double4* gx, double4* gv, double* rbar,
                            double4* gpsi, uint32_t* dup, double4* gx_out, double4* gv_out, double* nu_part,
-                           double* nu_bar, uint32_t n, const SimConsts& c, const DevObject* objs, hipStream_t s);
+                           double* nu_bar, uint32_t n, const SimConsts& c, const DevObject* objs, hipStream_t s,
+                           StageTimer* timer);
 int launch_grid_density(const float* origin, const float* spacing, const uint32_t* dims, const float4* xs,
                         const uint32_t* cell_start, const SimConsts& c, double* out, hipStream_t s);
 int launch_grid_loss_grad(const float* origin, const float* spacing, const uint32_t* dims, const float4* x,
@@ -77,7 +78,7 @@ void free_adjoint(vfs_adjoint* a) {
 // permutation and neighbour lists into a tape slot.  It sorts as the
 // simulation's own substeps do: from scratch the first time, then by
 // re-sorting the keys that changed bucket.
-int record_taped_substep(vfs_sim* sim, const vfs_adjoint::Slot& slot, hipStream_t s) {
+int record_taped_substep(vfs_sim* sim, const vfs_adjoint::Slot& slot, hipStream_t s, StageTimer* timer) {
     const uint32_t n = sim->n;
     StepBuffers b = sim->buffers();
     b.perm = slot.perm;
@@ -92,19 +93,27 @@ int record_taped_substep(vfs_sim* sim, const vfs_adjoint::Slot& slot, hipStream_
         // pair whatever state was restored since); the merge moves x and v
         // into the slot's sorted arrays
         VFS_TRY(resort_keys_iota(sim->plan, sim->keys, sim->sorted_keys, slot.perm, sim->cell_start, sim->sort_ws, s,
-                                 nullptr, &b));
+                                 timer, &b));
         sim->last_full_sort = false;
     } else {
-        VFS_TRY(sort_keys_iota(sim->plan, sim->keys, sim->sorted_keys, slot.perm, sim->sort_ws, s, nullptr,
+        VFS_TRY(sort_keys_iota(sim->plan, sim->keys, sim->sorted_keys, slot.perm, sim->sort_ws, s, timer,
                                sim->offsets, (size_t)n + 1, n));
+        if (timer) timer->begin(s, "reorder");
         VFS_TRY(launch_reorder_state(b, s));
+        if (timer) timer->end(s);
+        if (timer) timer->begin(s, "cell_ranges");
         VFS_TRY(launch_cell_start_scan(sim->plan, sim->offsets, sim->cell_start, sim->sort_ws, s));
+        if (timer) timer->end(s);
         sim->resort_ready = true;
         sim->last_full_sort = true;
     }
     const SimConsts& c = sim->scenes[0].consts;
+    if (timer) timer->begin(s, "density");
     VFS_TRY(launch_density(b, c, nullptr, sim->objs_dev, sim->tuning.block_size, sim->tuning.stage_rows, s));
+    if (timer) timer->end(s);
+    if (timer) timer->begin(s, "forces_integrate");
     VFS_TRY(launch_forces_integrate(b, c, nullptr, sim->tuning.block_size, s));
+    if (timer) timer->end(s);
     return 0;
 }
 
@@ -245,7 +254,14 @@ int vfs_adjoint_forward(vfs_adjoint* a, const float* x0_dev, const float* v0_dev
     return 0;
 }
 
-int vfs_adjoint_backward(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev, double* grad_viscosity) {
+}  // extern "C"
+
+namespace {
+
+// The reverse pass; timer (may be null) gets an event pair around every
+// launch group.
+int backward_impl(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev, double* grad_viscosity,
+                  StageTimer* timer) {
     VFS_CHECK(a, VFS_ERR_INVALID, "adjoint is null");
     VFS_CHECK(a->forward_done, VFS_ERR_STATE, "vfs_adjoint_forward has not run");
     vfs_sim* sim = a->sim;
@@ -254,8 +270,10 @@ int vfs_adjoint_backward(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev,
     const SimConsts& c = sim->scenes[0].consts;
     const uint32_t n_seg = (a->T + a->C - 1) / a->C;
     // d loss / d x of the final state (its array order), zero velocity adjoint
+    if (timer) timer->begin(s, "loss_gradient");
     VFS_TRY(launch_grid_loss_grad(a->origin, a->spacing, a->dims, a->ck_x[n_seg], n, a->rho_grid, a->target, c,
                                   a->gx_a, s));
+    if (timer) timer->end(s);
     VFS_TRY(launch_zero_d4(a->gv_a, n, s));
     VFS_HIP(hipMemsetAsync(a->nu_bar, 0, sizeof(double), s));
     double4 *gx_cur = a->gx_a, *gv_cur = a->gv_a, *gx_nxt = a->gx_b, *gv_nxt = a->gv_b;
@@ -266,9 +284,11 @@ int vfs_adjoint_backward(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev,
         VFS_TRY(copy_d2d(sim->v, a->ck_v[seg], (size_t)n * 16, s));
         sim->keys_valid = false;
         VFS_TRY(sim->ensure_keys(s));
-        for (uint32_t k = 0; k < steps; k++) VFS_TRY(record_taped_substep(sim, a->tape[k], s));
+        for (uint32_t k = 0; k < steps; k++) VFS_TRY(record_taped_substep(sim, a->tape[k], s, timer));
         VFS_HIP(hipMemsetAsync(a->maxc, 0, 4, s));
+        if (timer) timer->begin(s, "list_check");
         for (uint32_t k = 0; k < steps; k++) VFS_TRY(launch_max_u32(a->tape[k].cnt, n, a->maxc, s));
+        if (timer) timer->end(s);
         uint32_t maxc = 0;
         VFS_HIP(hipMemcpyAsync(&maxc, a->maxc, 4, hipMemcpyDeviceToHost, s));
         VFS_HIP(hipStreamSynchronize(s));
@@ -278,7 +298,7 @@ int vfs_adjoint_backward(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev,
             const auto& t = a->tape[k];
             VFS_TRY(launch_adjoint_substep(t.perm, t.xs, t.vs, t.rho, t.bnd, t.nbr, t.cnt, gx_cur, gv_cur, a->abar,
                                            a->gx, a->gv, a->rbar, a->gpsi, a->dup, gx_nxt, gv_nxt, a->nu_part,
-                                           a->nu_bar, n, c, sim->objs_dev, s));
+                                           a->nu_bar, n, c, sim->objs_dev, s, timer));
             std::swap(gx_cur, gx_nxt);
             std::swap(gv_cur, gv_nxt);
         }
@@ -293,6 +313,49 @@ int vfs_adjoint_backward(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev,
     sim->keys_valid = false;
     VFS_HIP(hipStreamSynchronize(s));
     if (grad_viscosity) *grad_viscosity = nb;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vfs_adjoint_backward(vfs_adjoint* a, float* grad_x0_dev, float* grad_v0_dev, double* grad_viscosity) {
+    return backward_impl(a, grad_x0_dev, grad_v0_dev, grad_viscosity, nullptr);
+}
+
+int vfs_adjoint_profile_backward(vfs_adjoint* a, float* ms, uint32_t* launches, const char** names,
+                                 uint32_t max_stages, uint32_t* n_stages) {
+    VFS_CHECK(a && n_stages, VFS_ERR_INVALID, "null argument");
+    *n_stages = 0;
+    static const char* kNames[] = {"loss_gradient", "sort_prepare", "sort_scan",   "sort_histogram", "sort_small",
+                                   "sort_pass",     "sort_merge",   "reorder",     "cell_ranges",    "density",
+                                   "forces_integrate", "list_check", "adj_integrate", "adj_forces",
+                                   "adj_viscosity_sum", "adj_boundary", "adj_density"};
+    constexpr uint32_t kN = sizeof(kNames) / sizeof(kNames[0]);
+    EventTimer timer;
+    VFS_TRY(backward_impl(a, nullptr, nullptr, nullptr, &timer));  // synchronises the stream
+    if (timer.err != hipSuccess) return fail(VFS_ERR_HIP, "event creation failed");
+    double tot[kN] = {0};
+    uint32_t cnt[kN] = {0};
+    for (auto& r : timer.recs) {
+        float t = 0.0f;
+        VFS_HIP(hipEventElapsedTime(&t, r.a, r.b));
+        for (uint32_t i = 0; i < kN; i++)
+            if (r.name == kNames[i]) {
+                tot[i] += t;
+                cnt[i]++;
+            }
+    }
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < kN && m < max_stages; i++) {
+        if (cnt[i] == 0) continue;
+        if (ms) ms[m] = (float)tot[i];
+        if (launches) launches[m] = cnt[i];
+        if (names) names[m] = kNames[i];
+        m++;
+    }
+    *n_stages = m;
     return 0;
 }
 

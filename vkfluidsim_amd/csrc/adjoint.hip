@@ -812,25 +812,37 @@ int launch_adjoint_substep(const uint32_t* perm, const float4* xs, const float4*
                            const float4* bnd, const uint32_t* nbr, const uint32_t* cnt, const double4* gx_in,
                            const double4* gv_in, double4* abar, double4* gx, double4* gv, double* rbar,
                            double4* gpsi, uint32_t* dup, double4* gx_out, double4* gv_out, double* nu_part,
-                           double* nu_bar, uint32_t n, const SimConsts& c, const DevObject* objs, hipStream_t s) {
+                           double* nu_bar, uint32_t n, const SimConsts& c, const DevObject* objs, hipStream_t s,
+                           StageTimer* timer) {
     if (n == 0) return 0;
     const TapeView t{perm, xs, vs, rho, bnd, nbr, cnt};
     const AdjBuffers a{gx_in, gv_in, abar, gx, gv, rbar, gpsi, dup, gx_out, gv_out, nu_part, n};
     const uint32_t nb = blocks_for(n, 256);
     const uint32_t nb_lanes = blocks_for((uint64_t)n * kAdjLanes, 256);  // four lanes per particle
     const AdjConsts k = adj_consts(c);
+    if (timer) timer->begin(s, "adj_integrate");
     hipLaunchKernelGGL(k_adj_integrate, dim3(nb), dim3(256), 0, s, t, a, c, k);
     VFS_TRY(check_launch("adj_integrate"));
+    if (timer) timer->end(s);
+    if (timer) timer->begin(s, "adj_forces");
     hipLaunchKernelGGL(k_adj_forces, dim3(nb_lanes), dim3(256), 0, s, t, a, c, k);
     VFS_TRY(check_launch("adj_forces"));
+    if (timer) timer->end(s);
+    if (timer) timer->begin(s, "adj_viscosity_sum");
     hipLaunchKernelGGL(k_add_partials, dim3(1), dim3(256), 0, s, nu_part, nb_lanes, nu_bar);
     VFS_TRY(check_launch("add_partials"));
+    if (timer) timer->end(s);
     if (c.obj_count) {
+        if (timer) timer->begin(s, "adj_boundary");
         hipLaunchKernelGGL(k_adj_boundary, dim3(nb), dim3(256), 0, s, t, a, c, k, objs);
         VFS_TRY(check_launch("adj_boundary"));
+        if (timer) timer->end(s);
     }
+    if (timer) timer->begin(s, "adj_density");
     hipLaunchKernelGGL(k_adj_density, dim3(nb_lanes), dim3(256), 0, s, t, a, c, k);
-    return check_launch("adj_density");
+    VFS_TRY(check_launch("adj_density"));
+    if (timer) timer->end(s);
+    return 0;
 }
 
 int launch_grid_density(const float* origin, const float* spacing, const uint32_t* dims, const float4* xs,

@@ -178,34 +178,6 @@ int validate_objects(const vfs_boundary_object* objects, uint32_t n_objects) {
 
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
-struct EventTimer : StageTimer {
-    struct Rec {
-        std::string name;
-        hipEvent_t a, b;
-    };
-    std::vector<Rec> recs;
-    std::string pending;
-    hipEvent_t pa = nullptr;
-    hipError_t err = hipSuccess;
-    void begin(hipStream_t s, const char* name) override {
-        pending = name;
-        if (hipEventCreate(&pa) != hipSuccess) err = hipErrorUnknown;
-        (void)hipEventRecord(pa, s);
-    }
-    void end(hipStream_t s) override {
-        hipEvent_t pb;
-        if (hipEventCreate(&pb) != hipSuccess) err = hipErrorUnknown;
-        (void)hipEventRecord(pb, s);
-        recs.push_back({pending, pa, pb});
-    }
-    ~EventTimer() override {
-        for (auto& r : recs) {
-            (void)hipEventDestroy(r.a);
-            (void)hipEventDestroy(r.b);
-        }
-    }
-};
-
 // Builds a simulation of n_scenes scenes (objects listed scene after scene,
 // objects_per_scene[s] of them for scene s).
 template <class T>

@@ -21,6 +21,36 @@ struct SceneSeg {
     float cell0 = 0.0f;   // the scene's own cell size (cell_size, or smooth_radius)
 };
 
+// A HIP event pair around every timed launch group, kept with its stage name
+// until the caller has synchronised and read the elapsed times.
+struct EventTimer : StageTimer {
+    struct Rec {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::string pending;
+    hipEvent_t pa = nullptr;
+    hipError_t err = hipSuccess;
+    void begin(hipStream_t s, const char* name) override {
+        pending = name;
+        if (hipEventCreate(&pa) != hipSuccess) err = hipErrorUnknown;
+        (void)hipEventRecord(pa, s);
+    }
+    void end(hipStream_t s) override {
+        hipEvent_t pb;
+        if (hipEventCreate(&pb) != hipSuccess) err = hipErrorUnknown;
+        (void)hipEventRecord(pb, s);
+        recs.push_back({pending, pa, pb});
+    }
+    ~EventTimer() override {
+        for (auto& r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+};
+
 }  // namespace vfs
 
 using namespace vfs;
