@@ -116,3 +116,37 @@ def test_random_scene_bit_exact(seed):
         sim.substeps(8)
         orc.substeps(8)
         _assert_same(sim, orc, f"case {seed} after {8 * (k + 1)} substeps")
+
+
+def test_random_scenes_as_one_batch_match_single_scenes_and_the_oracle():
+    """The first 16 random scenes in one batch (each with its own smoothing
+    radius, hash cell, box, gravity, EOS and boundary objects), 16 substeps:
+    every scene equals its own vfs_sim bit for bit (state, hash tables,
+    boundary terms) and the oracle's run of it."""
+    cases = [_case(seed) for seed in range(16)]
+    batch = V.Batch([p for p, _, _, _ in cases], [g for _, g, _, _ in cases])
+    for k, (_, _, x, v) in enumerate(cases):
+        batch.set_state(k, x, v)
+    batch.substeps(16)
+    for k, (params, grids, x, v) in enumerate(cases):
+        print(f"scene {k}: n={params.n_particles}", flush=True)
+        sim = V.Sim(params, grids)
+        sim.set_state(x, v)
+        sim.substeps(16)
+        for a, b in zip(batch.get_state(k, ids=True), sim.get_state(ids=True)):
+            np.testing.assert_array_equal(a, b, err_msg=f"scene {k}: state")
+        for a, b in zip(batch.get_hash(k), sim.get_hash()):
+            np.testing.assert_array_equal(a, b, err_msg=f"scene {k}: hash")
+        for a, b in zip(batch.get_boundary(k), sim.get_boundary()):
+            np.testing.assert_array_equal(a, b, err_msg=f"scene {k}: boundary")
+        sim.close()
+        orc = H.OracleSim(params, grids)
+        orc.set_state(x, v)
+        orc.substeps(16)
+        bx, bv, brho, bpid = batch.get_state(k, ids=True)
+        rx, rv, rrho, rpid, _ = orc.get_state()
+        np.testing.assert_array_equal(bpid, rpid, err_msg=f"scene {k}: ids vs oracle")
+        np.testing.assert_array_equal(bx.astype(np.float64), rx, err_msg=f"scene {k}: x vs oracle")
+        np.testing.assert_array_equal(bv.astype(np.float64), rv, err_msg=f"scene {k}: v vs oracle")
+        np.testing.assert_array_equal(brho.astype(np.float64), rrho, err_msg=f"scene {k}: rho vs oracle")
+    batch.close()
