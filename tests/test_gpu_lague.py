@@ -176,3 +176,42 @@ def test_cpp_dam_break_scene_matches_the_c_abi(tmp_path):
     np.testing.assert_array_equal(v, ev)
     np.testing.assert_array_equal(rho, er)
     sim.close()
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_lague_random_scenes_bit_exact(seed):
+    """Seeded random Lague scenes: ragged particle counts (30-20,000), a
+    jittered block plus a cloud reaching past the box (offset into negative
+    coordinates), random velocities and gravity, smoothing radius, rest
+    density, time step and the four model parameters; 20 substeps, bit-exact
+    against the oracle after every 5."""
+    r = np.random.default_rng(9100 + seed)
+    n = int(np.exp(r.uniform(np.log(30.0), np.log(20000.0))))
+    h = float(r.uniform(0.1, 0.4))
+    box = r.uniform(2.0, 8.0, 3)
+    box_pos = r.uniform(-3.0, 0.5, 3)
+    rho0 = float(r.uniform(300.0, 1000.0))
+    spacing = h / r.uniform(1.5, 2.5)
+    n_block = int(n * r.uniform(0.5, 0.9))
+    side = int(np.ceil(n_block ** (1.0 / 3.0)))
+    block = CF.lattice((side, side, side), spacing, box_pos + spacing, 0.1 * spacing, 300 + seed)[:n_block]
+    cloud = (box_pos + r.uniform(-0.1, 1.1, (n - n_block, 3)) * box).astype(np.float32)
+    x = np.concatenate([block, cloud]).astype(np.float32)
+    v = r.standard_normal((n, 3)).astype(np.float32)
+    g = r.standard_normal(3)
+    p = V.make_params(n, h, dt=float(r.uniform(0.002, 0.01)), box_size=box, box_pos=box_pos, rho0=rho0,
+                      gravity=9.81 * g / np.linalg.norm(g), iterations=int(r.integers(1, 4)),
+                      flags=V.VFS_FLAG_TRACK_IDS)
+    lp = V.make_lague_params(wall_damping_factor=float(r.uniform(0.5, 1.0)),
+                             pressure_multiplier=float(r.uniform(50.0, 500.0)),
+                             near_pressure_multiplier=float(r.uniform(0.5, 5.0)),
+                             viscosity=float(r.uniform(0.0, 0.1)))
+    print(f"case lague random {seed}: n={n} h={h:.3f} rho0={rho0:.0f}", flush=True)
+    sim = V.Lague(p, lp)
+    sim.set_state(x, v)
+    orc = H.OracleLague(p, lp)
+    orc.set_state(x, v)
+    for _ in range(4):
+        sim.substeps(5)
+        orc.substeps(5)
+        _same(sim, orc)
