@@ -213,3 +213,55 @@ def test_backward_accepts_lists_at_the_capacity_edge():
     assert np.all(np.isfinite(gx)) and np.all(np.isfinite(gv)) and np.isfinite(gnu)
     adj.close()
     sim.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_adjoint_random_scenes_match_fp64_adjoint_of_the_same_trajectory(seed):
+    """Seeded random scenes for the reverse pass: a jittered block of 300-4,000
+    particles at about half the smoothing radius apart, random velocities,
+    box (offset into negative coordinates), gravity, stiffness, exponent and
+    viscosity, box walls and sometimes a translated sphere, 8-30 substeps with
+    a random checkpoint interval and grid.  The GPU gradients equal the FP64
+    adjoint of the same FP32 trajectory to 1e-6, as in the fixed case."""
+    r = np.random.default_rng(4400 + seed)
+    h = float(r.uniform(0.03, 0.06))
+    box = r.uniform(0.5, 1.2, 3)
+    box_pos = r.uniform(-0.5, 0.2, 3)
+    side = tuple(int(v) for v in r.integers(6, 16, 3))
+    spacing_p = h / r.uniform(1.9, 2.2)
+    x = CF.lattice(side, spacing_p, box_pos + 2 * spacing_p, 0.05 * spacing_p, 500 + seed)
+    n = x.shape[0]
+    v = (0.2 * r.standard_normal((n, 3))).astype(np.float32)
+    g = r.standard_normal(3)
+    steps = int(r.integers(8, 31))
+    every = int(r.integers(1, steps + 1))
+    p = V.make_params(n, h, dt=float(r.uniform(0.5, 2.0) * 1e-4), box_size=box, box_pos=box_pos,
+                      stiffness=float(r.uniform(200.0, 2000.0)), gamma=float(r.choice([1.0, 7.0])),
+                      viscosity=float(10.0 ** r.uniform(-3.0, -1.0)), gravity=9.81 * g / np.linalg.norm(g))
+    c = box_pos + 0.5 * box
+    grids = [V.volume_map(V.VFS_SHAPE_BOX_WALLS, (*c, *(0.5 * box)), (int(r.integers(16, 33)),) * 3, h)]
+    if r.random() < 0.5:
+        sphere = V.volume_map(V.VFS_SHAPE_SPHERE, (0.0, 0.0, 0.0, float(r.uniform(0.05, 0.15))), (20, 20, 20), h)
+        t = np.eye(4, dtype=np.float32)
+        t[3, :3] = -(box_pos + r.uniform(0.3, 0.7, 3) * box).astype(np.float32)
+        sphere.transform = t.reshape(-1)
+        grids.append(sphere)
+    dims = tuple(int(d) for d in r.integers(8, 24, 3))
+    spacing = tuple(float(b) / (d - 1) for b, d in zip(box, dims))
+    origin = tuple(float(o) for o in box_pos)
+    print(f"case adjoint random {seed}: n={n} steps={steps} every={every} objects={len(grids)}", flush=True)
+    sim = V.Sim(p, grids)
+    adj = V.Adjoint(sim, steps, every, origin, spacing, dims)
+    _, target = adj.forward(x, v + (0.01 * r.standard_normal((n, 3))).astype(np.float32))
+    loss, _ = adj.forward(x, v, target)
+    gx, gv, gnu = adj.backward()
+    L, mx, mv, mnu = H.oracle_adjoint_f32_trajectory(p, grids, x, v, steps, origin, spacing, dims,
+                                                     target.astype(np.float64))
+    ex = np.linalg.norm(gx - mx) / np.linalg.norm(mx)
+    ev = np.linalg.norm(gv - mv) / np.linalg.norm(mv)
+    print(f"  x0 {ex:.2e}, v0 {ev:.2e}, viscosity {gnu:.8e} vs {mnu:.8e}")
+    assert loss == pytest.approx(L, rel=1e-9)
+    assert ex < 1e-6 and ev < 1e-6
+    assert gnu == pytest.approx(mnu, rel=1e-6, abs=1e-12 * max(1.0, abs(mnu)))
+    adj.close()
+    sim.close()
