@@ -446,6 +446,45 @@ def bench_adjoint(steps: int, every: int, repeats: int = 3) -> dict:
     }
 
 
+def bench_reference_scenes(frames: int) -> dict:
+    """The reference's own scenes, as a user of it runs them: assets/scenes/
+    dam_break.json (75,000 particles) and obstacle.json (40,000 particles,
+    box.obj rotated and scaled, suzanne.obj), loaded from their committed
+    copies (tests/golden/reference_scene) and stepped frame by frame as
+    GenericScene::Step does (one frame = `iterations` substeps).  Reports the
+    set-up time (JSON, seeding, every boundary object's SDF and volume map on
+    the GPU) and the time per frame."""
+    import vkfluidsim_amd as V
+    from vkfluidsim_amd._lib import call
+
+    root = ROOT / "tests" / "golden" / "reference_scene"
+    out = {}
+    for name in ("dam_break", "obstacle"):
+        t0 = time.perf_counter()
+        sc = V.Scene(str(root / f"{name}.json"), str(root))
+        call("vfs_sim_synchronize", sc.sim_handle)
+        t1 = time.perf_counter()
+        sc.step()  # first frame: full sort, graph capture
+        call("vfs_sim_synchronize", sc.sim_handle)
+        t2 = time.perf_counter()
+        for _ in range(frames):
+            sc.step()
+        call("vfs_sim_synchronize", sc.sim_handle)
+        t3 = time.perf_counter()
+        it = int(sc.params.iterations)
+        out[name] = {
+            "particles": sc.n,
+            "iterations_per_frame": it,
+            "frames": frames,
+            "setup_ms": round((t1 - t0) * 1e3, 1),
+            "ms_per_frame": (t3 - t2) * 1e3 / frames,
+            "frames_per_s": frames / (t3 - t2),
+            "particle_steps_per_s": sc.n * it * frames / (t3 - t2),
+        }
+        sc.close()
+    return out
+
+
 def bench_lague(steps: int, cpu_steps: int, with_cpu: bool) -> dict:
     """The reference's Lague dam break (dam_break_scene.cpp:8-50): 150,000
     particles, target density 630, 23 x 10 x 10 box, smooth radius 0.2, three
@@ -716,6 +755,8 @@ def main() -> int:
     ap.add_argument("--looped-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward + adjoint measurement")
     ap.add_argument("--no-lague", action="store_true", help="skip the Lague model measurement")
+    ap.add_argument("--no-reference-scenes", action="store_true",
+                    help="skip the reference's own dam_break.json and obstacle.json scenes")
     ap.add_argument("--no-extra-legs", action="store_true", help="skip the free-fall and sort-stress legs")
     ap.add_argument("--no-free-fall", action="store_true", help="skip the free-fall leg")
     ap.add_argument("--no-sort-stress", action="store_true", help="skip the 8M sort-stress leg")
@@ -868,6 +909,7 @@ def main() -> int:
                                                                   0 if args.no_looped else args.looped_steps)
     out["adjoint"] = None if args.no_adjoint else bench_adjoint(args.adjoint_steps, args.adjoint_every)
     out["lague"] = None if args.no_lague else bench_lague(args.lague_steps, 100, not args.no_cpu_baseline)
+    out["reference_scenes"] = None if args.no_reference_scenes else bench_reference_scenes(100)
     out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(w, grids, x0, v0, args.cpu_steps)
     tmpdir.cleanup()
     print(json.dumps(out))
