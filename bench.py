@@ -547,6 +547,22 @@ def summarize_stats(db: str) -> str:
     return "\n".join(lines)
 
 
+def summarize_stats_by_grid(db: str) -> str:
+    """Per kernel and grid size (threads) of a rocprofv3 --kernel-trace
+    database: calls, total and mean duration (the kernels view, in ns; printed
+    in ms), so that runs of several scene sizes in one process separate."""
+    import sqlite3
+
+    rows = sqlite3.connect(db).execute(
+        "select name, grid_x, count(*), sum(duration), avg(duration) from kernels group by name, grid_x "
+        "order by sum(duration) desc").fetchall()
+    lines = [f"{'kernel':34s} {'threads':>9s} {'calls':>6s} {'total_ms':>11s} {'avg_ms':>9s}"]
+    for name, grid, calls, total, avg in rows:
+        short = name.replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0].replace("vfs::", "")
+        lines.append(f"{short:34s} {grid:9d} {calls:6d} {total / 1e6:11.1f} {avg / 1e6:9.3f}")
+    return "\n".join(lines)
+
+
 SQ_COUNTER_PASSES = (
     "SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_ANY SQ_ACTIVE_INST_VALU "
     "SQ_ACTIVE_INST_LDS",
@@ -769,11 +785,16 @@ def main() -> int:
     ap.add_argument("--state", metavar="NPZ", help="settled state of the workload written by --make-state "
                                                    "(default: made by a child of this run)")
     ap.add_argument("--summarize-stats", metavar="DB", help="print a rocprofv3 --stats database as text")
+    ap.add_argument("--summarize-stats-by-grid", metavar="DB",
+                    help="print a rocprofv3 --kernel-trace database per kernel and grid size")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DB", "WRITE_DB"),
                     help="print the HBM traffic of two --pmc databases of --traffic-probe runs")
     ap.add_argument("--summarize-counters", nargs="+", metavar="DB",
                     help="print the per-wave SQ counters of --pmc databases of --traffic-probe runs")
     args = ap.parse_args()
+    if args.summarize_stats_by_grid:
+        print(summarize_stats_by_grid(args.summarize_stats_by_grid))
+        return 0
     if args.summarize_stats:
         print(summarize_stats(args.summarize_stats))
         return 0

@@ -97,3 +97,20 @@ def test_counter_summary_memory_path_fractions(tmp_path):
     assert "TD_TC_STALL_sum per CU / kernel cycles: 0.400" in text
     assert "per CU / kernel cycles: 0.250" in text  # 16 TD cycles per VMEM read instruction
     assert "L1 hit rate (1 - TCP_TCC_READ_REQ / TCP_TOTAL_CACHE_ACCESSES): 0.800" in text
+
+
+def test_stats_by_grid_separates_scene_sizes(tmp_path):
+    """summarize_stats_by_grid groups a --kernel-trace database's dispatches
+    by kernel and grid size (durations in ns, printed in ms)."""
+    import sqlite3
+
+    db = tmp_path / "k.db"
+    con = sqlite3.connect(db)
+    con.execute("create table kernels (name text, grid_x integer, duration integer)")
+    dens = "void vfs::(anonymous namespace)::k_density<128, true, false>(vfs::StepBuffers, ...)"
+    rows = [(dens, 75776, 80000)] * 3 + [(dens, 40960, 60000), (dens, 40960, 62000)]
+    con.executemany("insert into kernels values (?, ?, ?)", rows)
+    con.commit()
+    lines = bench.summarize_stats_by_grid(str(db)).splitlines()
+    assert lines[1].split() == ["k_density<128,", "true,", "false>", "75776", "3", "0.2", "0.080"]
+    assert lines[2].split() == ["k_density<128,", "true,", "false>", "40960", "2", "0.1", "0.061"]
