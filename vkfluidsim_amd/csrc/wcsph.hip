@@ -384,7 +384,12 @@ __device__ __forceinline__ void force_pair(ForceAcc& f, const float4& p4, const 
 // over this substep's neighbour list (one walk, separate accumulators), then
 // WallAcceleration and UpdatePositions (slang:336-360), then UpdateSpatialHash
 // of the new position.  Particles whose list overflowed walk their cells.
-template <int BLOCK, bool BATCH>
+// DEEP: each list chunk's indices are loaded two chunks ahead of its
+// neighbours' gathers, so that no iteration waits for an index load and then
+// for the gathers it feeds.  It costs registers, so it is launched where the
+// particles fill about a wave per SIMD and latency, not occupancy, bounds the
+// pass (the same pairs in the same order: results are identical).
+template <int BLOCK, bool BATCH, bool DEEP = false>
 __global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimConsts c0, SceneTable table,
                                                             uint32_t nblocks) {
     const uint32_t i = xcd_block(nblocks) * (uint32_t)BLOCK + threadIdx.x;
@@ -409,12 +414,20 @@ __global__ __launch_bounds__(BLOCK) void k_forces_integrate(StepBuffers b, SimCo
         const uint4* __restrict__ list = reinterpret_cast<const uint4*>(b.nbr) + i;
         const uint32_t groups = (count + 3u) >> 2;
         uint4 e = groups ? list[0] : make_uint4(0u, 0u, 0u, 0u);
+        uint4 e1 = make_uint4(0u, 0u, 0u, 0u);
+        if (DEEP && groups) e1 = list[(size_t)min(1u, groups - 1u) * b.n];
         float4 q0 = load_f4(xr, e.x), w0 = load_f4(vr, e.x);
         float4 q1 = load_f4(xr, e.y), w1 = load_f4(vr, e.y);
         float4 q2 = load_f4(xr, e.z), w2 = load_f4(vr, e.z);
         float4 q3 = load_f4(xr, e.w), w3 = load_f4(vr, e.w);
         for (uint32_t g = 0; g < groups; g++) {
-            const uint4 en = list[(size_t)min(g + 1u, groups - 1u) * b.n];
+            uint4 en;
+            if (DEEP) {
+                en = e1;
+                e1 = list[(size_t)min(g + 2u, groups - 1u) * b.n];
+            } else {
+                en = list[(size_t)min(g + 1u, groups - 1u) * b.n];
+            }
             const float4 n0 = load_f4(xr, en.x), m0 = load_f4(vr, en.x);
             const float4 n1 = load_f4(xr, en.y), m1 = load_f4(vr, en.y);
             const float4 n2 = load_f4(xr, en.z), m2 = load_f4(vr, en.z);
@@ -790,6 +803,10 @@ int launch_density(const StepBuffers& b, const SimConsts& c, const SceneTable* t
     return check_launch("density");
 }
 
+// Single scenes up to this many particles (about two waves per SIMD of the
+// 1,024) take the force pass with list indices loaded two chunks ahead.
+constexpr uint32_t kForcesDeepMax = 131072;
+
 int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, const SceneTable* table, int block,
                             hipStream_t s) {
     if (b.n == 0) return 0;
@@ -801,6 +818,11 @@ int launch_forces_integrate(const StepBuffers& b, const SimConsts& c, const Scen
             hipLaunchKernelGGL((k_forces_integrate<128, true>), dim3(nb), dim3(128), 0, s, b, c, t, nb);
         else
             hipLaunchKernelGGL((k_forces_integrate<256, true>), dim3(nb), dim3(256), 0, s, b, c, t, nb);
+    } else if (b.n <= kForcesDeepMax) {
+        if (bs == 128)
+            hipLaunchKernelGGL((k_forces_integrate<128, false, true>), dim3(nb), dim3(128), 0, s, b, c, t, nb);
+        else
+            hipLaunchKernelGGL((k_forces_integrate<256, false, true>), dim3(nb), dim3(256), 0, s, b, c, t, nb);
     } else {
         if (bs == 128)
             hipLaunchKernelGGL((k_forces_integrate<128, false>), dim3(nb), dim3(128), 0, s, b, c, t, nb);
